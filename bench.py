@@ -1,0 +1,183 @@
+"""Benchmark of the statevector hot path on one MI355X.
+
+Workload (default): 30 qubits, complex128 (16 GiB state), depth 40; per layer a U3 on every qubit
+(angles from a seeded PCG64), a CX brickwork (15 even + 14 odd pairs) and 15 CRZ gates on random
+ordered qubit pairs: 2,960 gates. One step = the whole circuit applied to the HBM-resident state.
+
+Printed (one JSON line):
+  value          HBM GB/s in the pass model: sum over kernel launches of the bytes each launch must
+                 read + write (2 * 16 B * amplitudes it touches) / wall time of the step
+  roofline       the dominant kernel's algorithmic bytes per launch / its mean HIP-event duration,
+                 against MI355X's 8 TB/s HBM peak
+  gate_equiv     the unfused per-gate byte model (sum over gates of 2 * 16 B * 2^(n - controls)) /
+                 step time: comparable with the CPU, may exceed the HBM peak once gates are fused
+  cpu_baseline   the CPU oracle (a restatement of qfvm's OpenMP engine, built with its flags) on
+                 the first layer of the same circuit, same unit as gate_equiv
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "HBM GB/s (% peak) on 30q c128 gates; adjoint grads/sec vs param-shift, 1 GPU"
+PEAK_GBPS = 8000.0
+
+
+def gate_equiv_bytes(prog, n: int, amp_bytes: int) -> float:
+    """Sum over gates of 2 * s * 2^(n - controls) (the reference's one-sweep-per-gate model)."""
+    from pyquafu_amd.simulators import program as P
+
+    ctrl_of = {P.CX: 1, P.CZ: 1, P.CP: 1, P.CCX: 2}
+    tot = 0.0
+    for r in prog.ops:
+        code = int(r["code"])
+        if code in (P.MEASURE, P.RESET, P.CIF):
+            continue
+        if code == P.RZZ:
+            tot += 2 * amp_bytes * (2 * 2.0 ** (n - 1) + 2.0**n)
+            continue
+        c = int(r["n_ctrl"]) if code in (P.GENERAL, P.MULTI) else ctrl_of.get(code, 0)
+        tot += 2 * amp_bytes * 2.0 ** (n - c)
+    return tot
+
+
+def cpu_baseline(prog, n: int, layer_gates: int, amp_bytes: int):
+    """Oracle (CPU restatement of qfvm) on the first `layer_gates` gates of the program."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle.cpu import CpuState
+    from pyquafu_amd.simulators.program import Program
+
+    sub = Program(prog.ops[:layer_gates], prog.idx, prog.mat, n_qubits=n)
+    s = CpuState(n)
+    t0 = time.perf_counter()
+    s.apply(sub)
+    dt = time.perf_counter() - t0
+    by = gate_equiv_bytes(sub, n, 16)
+    del s
+    return {
+        "value": round(by / dt / 1e9, 3),
+        "unit": "GB/s gate-equivalent",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {layer_gates} gates (one layer) of the same {n}-qubit circuit, complex128, "
+                  f"{dt:.2f} s, one OpenMP sweep per gate",
+        "seconds": round(dt, 3),
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--crz", type=int, default=15)
+    ap.add_argument("--dtype", default="c128", choices=["c128", "c64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the statevector "
+              "engine holds the whole state on one MI355X; no multi-GPU figure is produced)", file=sys.stderr)
+        return 2
+
+    from pyquafu_amd import _native, workloads
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    n = args.n
+    amp = 16 if args.dtype == "c128" else 8
+    qc = workloads.random_u3_cx(n, args.depth, args.crz, seed=20250002, measure=False)
+    prog = compile_circuit(qc)
+    print(f"[bench] {n}q {args.dtype} depth {args.depth}: {prog.n_ops} gates", file=sys.stderr, flush=True)
+
+    dev = _native.DeviceState(n, args.dtype)
+    dev.init_zero()
+    for k in range(args.warmup):
+        dev.apply(prog)
+        print(f"[bench] warmup {k + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    dev.synchronize()
+    dev.profile(reset=True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        dev.apply(prog)
+        print(f"[bench] step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    dev.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    counters = dev.profile(reset=True)
+    pass_bytes = sum(v["bytes"] for v in counters.values()) / args.steps
+    launches = sum(v["launches"] for v in counters.values()) / args.steps
+
+    # one more step with HIP events around every launch, for the per-kernel roofline
+    dev.set_profiling(True)
+    dev.apply(prog)
+    dev.synchronize()
+    prof = dev.profile(reset=True)
+    dev.set_profiling(False)
+    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_bytes = dom["bytes"] / dom["launches"]
+    per_launch_s = dom["ms"] / dom["launches"] / 1e3
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    norm_err = abs(dev.norm2() - 1.0)
+
+    ge = gate_equiv_bytes(prog, n, amp)
+    out = {
+        "metric": METRIC,
+        "value": round(pass_bytes / dt / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic: seeded random U3 + CX brickwork + CRZ circuit (PCG64 seed 20250002), state from |0>",
+        "config": {
+            "workload": f"{n}q {args.dtype} random U3+CX brickwork+{args.crz} CRZ/layer, depth {args.depth}",
+            "n_qubits": n,
+            "depth": args.depth,
+            "gates": int(prog.n_ops),
+            "kernel_launches_per_step": launches,
+            "pass_bytes_per_step": pass_bytes,
+            "parallelism": "single GPU",
+        },
+        "frac_of_peak": round(pass_bytes / dt / 1e9 / PEAK_GBPS, 4),
+        "gate_equiv": {"GBps": round(ge / dt / 1e9, 2), "bytes_per_step": ge},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 2),
+            "peak": PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_GBPS, 4),
+            "traffic": None,
+            "bytes_per_launch": per_launch_bytes,
+            "mean_launch_ms": round(per_launch_s * 1e3, 4),
+        },
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in prof.items()},
+        "norm_error": norm_err,
+    }
+    dev.close()
+    if not args.no_cpu:
+        per_layer = n + n // 2 + (n - 1) // 2 + args.crz
+        print(f"[bench] CPU baseline: {per_layer} gates on the oracle", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(prog, n, per_layer, amp)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

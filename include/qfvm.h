@@ -1,0 +1,191 @@
+/*
+ * qfvm.h -- C-ABI of the MI355X statevector engine (libqfvm_hip.so).
+ *
+ * This is the drop-in boundary for pyquafu's qfvm statevector path. Every entry point replaces one
+ * piece of the reference's pybind module `qfvm` (src/qfvm/qfvm.cpp:263-288) or of the StateVector
+ * engine it drives (src/qfvm/statevector.hpp). Plain C: no C++ or torch types cross it, so ctypes,
+ * cffi, cgo or JNI can bind it directly.
+ *
+ * Conventions
+ *   - Qubit q is bit q of the amplitude index (little-endian statevector), as in the reference.
+ *   - Host state buffers are interleaved complex128 (re, im doubles), the layout of a numpy
+ *     complex128 array. The device state is complex128 or complex64 (QFVM_C128 / QFVM_C64).
+ *   - Every function returns a qfvm_status; qfvm_last_error() gives the message of the last
+ *     failure on the calling thread. The reference throws C++ exceptions instead
+ *     (src/qfvm/simulator.hpp:26, statevector.hpp:108-115); the Python layer maps the codes back
+ *     to the reference's exception types.
+ *   - A handle owns one device state and one HIP stream. Calls on one handle must not overlap.
+ */
+#ifndef QFVM_H
+#define QFVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qfvm_state* qfvm_handle;
+
+typedef enum qfvm_status {
+    QFVM_OK = 0,
+    QFVM_ERR_VALUE = 1,   /* bad argument (ValueError) */
+    QFVM_ERR_INDEX = 2,   /* qubit / cbit / size out of range (IndexError) */
+    QFVM_ERR_MEMORY = 3,  /* device allocation failed (MemoryError) */
+    QFVM_ERR_DEVICE = 4,  /* HIP runtime error or no device (RuntimeError) */
+    QFVM_ERR_RUNTIME = 5  /* any other failure (RuntimeError) */
+} qfvm_status;
+
+typedef enum qfvm_dtype { QFVM_C64 = 1, QFVM_C128 = 2 } qfvm_dtype;
+
+/*
+ * Op codes. The named codes are the reference's native gate set (OPMAP, src/qfvm/qasm.hpp:17-46),
+ * dispatched as in apply_op (src/qfvm/simulator.hpp:30-113). Every other gate reaches the engine
+ * as GENERAL (one target, 2x2 matrix, any number of controls; apply_op_general,
+ * simulator.hpp:12-28) or MULTI (k >= 2 targets, dense 2^k x 2^k matrix whose row-index bit j
+ * belongs to the j-th target; statevector.hpp:945-995).
+ */
+typedef enum qfvm_opcode {
+    QFVM_OP_GENERAL = 0,
+    QFVM_OP_MULTI = 1,
+    QFVM_OP_X = 2,
+    QFVM_OP_Y = 3,
+    QFVM_OP_Z = 4,
+    QFVM_OP_H = 5,
+    QFVM_OP_S = 6,
+    QFVM_OP_SDG = 7,
+    QFVM_OP_T = 8,
+    QFVM_OP_TDG = 9,
+    QFVM_OP_P = 10,
+    QFVM_OP_RX = 11,
+    QFVM_OP_RY = 12,
+    QFVM_OP_RZ = 13,
+    QFVM_OP_CX = 14, /* cx and cnot */
+    QFVM_OP_CZ = 15,
+    QFVM_OP_CP = 16,
+    QFVM_OP_CCX = 17,
+    QFVM_OP_RZZ = 18,
+    QFVM_OP_MEASURE = 19, /* qubits -> cbits, projective, consumes one uniform */
+    QFVM_OP_RESET = 20,   /* qubits -> |0>, consumes one uniform */
+    QFVM_OP_CIF = 21,     /* if creg[cbits] (big-endian) == cond: run the next `body` ops */
+    QFVM_OP_COUNT = 22
+} qfvm_opcode;
+
+/*
+ * One instruction of a flat program (64 bytes). Variable-length data lives in the program's pools:
+ *   qubits  idx[q_off .. q_off + n_qubits)   controls first, then targets (instructions.hpp:265)
+ *   cbits   idx[c_off .. c_off + n_cbits)    MEASURE / CIF only
+ *   matrix  mat[m_off .. m_off + 2*4^k)      row-major complex128, GENERAL (k=1) and MULTI only
+ */
+typedef struct qfvm_op {
+    int32_t code;
+    int32_t n_ctrl;
+    int32_t n_qubits;
+    int32_t n_cbits;
+    int64_t q_off;
+    int64_t c_off;
+    int64_t m_off;
+    double param; /* angle of P, RX, RY, RZ, CP, RZZ */
+    int64_t cond; /* CIF condition */
+    int64_t body; /* CIF: number of following ops in its body */
+} qfvm_op;
+
+typedef struct qfvm_program {
+    const qfvm_op* ops;
+    int64_t n_ops;
+    const int32_t* idx;
+    int64_t n_idx;
+    const double* mat;
+    int64_t n_mat;
+} qfvm_program;
+
+/* Last error message of the calling thread ("" if none). */
+const char* qfvm_last_error(void);
+
+/* Library / device information: HIP device count visible to this process. */
+int qfvm_device_count(int* out);
+
+/*
+ * Allocate a device state of n_qubits (0..34) and initialise it to |0...0>.
+ * Replaces StateVector(uint num) (statevector.hpp:148-152). Allocation failure -> QFVM_ERR_MEMORY.
+ */
+int qfvm_create(int32_t n_qubits, int32_t dtype, qfvm_handle* out);
+int qfvm_destroy(qfvm_handle h);
+int qfvm_num_qubits(qfvm_handle h, int32_t* out);
+int qfvm_dtype_of(qfvm_handle h, int32_t* out);
+
+/* |0...0> (statevector.hpp:149-152) and a computational basis state |index>. */
+int qfvm_init_zero(qfvm_handle h);
+int qfvm_init_basis(qfvm_handle h, uint64_t index);
+
+/*
+ * Copy a host complex128 state in / out (n_amps must be 2^n). load replaces adopting the caller's
+ * numpy buffer (StateVector::load_data, statevector.hpp:44-54 / qfvm.cpp:87-89); store is the
+ * device-to-host half of returning the state (qfvm.cpp:99-103). A complex64 device state is
+ * widened / rounded to nearest on the way.
+ */
+int qfvm_load(qfvm_handle h, const double* host, uint64_t n_amps);
+int qfvm_store(qfvm_handle h, double* host, uint64_t n_amps);
+
+/*
+ * Apply a unitary program (no MEASURE / RESET / CIF). Replaces simulate() over the gate list
+ * (simulator.hpp:115-125) and applyop_statevec for a one-op program (qfvm.cpp:36-50). Gates may
+ * be fused into multi-gate passes; the result equals the gate-by-gate product to rounding.
+ */
+int qfvm_apply(qfvm_handle h, const qfvm_program* prog);
+
+/*
+ * Run a program with mid-circuit semantics: MEASURE collapses (apply_measure,
+ * statevector.hpp:1342-1356), RESET projects to |0> (apply_reset, :1358-1362), CIF tests the
+ * classical register (check_cif, :186-193). Each MEASURE / RESET consumes the next entry of
+ * `uniforms` (values in [0,1)) in place of the reference's std::random_device draw. When
+ * skip_measure != 0, MEASURE ops are skipped (final-measure mode, simulator.hpp:119-122).
+ * creg has n_creg entries (0/1), zeroed by the caller.
+ */
+int qfvm_run(qfvm_handle h, const qfvm_program* prog, const double* uniforms, int64_t n_uniforms,
+             int32_t* creg, int32_t n_creg, int32_t skip_measure);
+
+/*
+ * Pauli-string expectations <psi|P_t|psi>, unscaled by coefficients. Replaces expect_statevec
+ * (qfvm.cpp:246-261) -> expect_pauli (statevector.hpp:1250-1305). Term t uses
+ * chars[off[t] .. off[t]+len[t]) ('I','X','Y','Z') on qubits pos[off[t] ..].
+ */
+int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
+                       const int32_t* len, int64_t n_terms, double* out);
+
+/* |amp|^2 for every basis state (probabilities(), statevector.hpp:1023-1032). */
+int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n);
+
+/*
+ * Sample basis-state indices, one per uniform u in [0,1). Replaces the per-shot CDF scan of
+ * measure_samples (statevector.hpp:1380-1412): the result is the basis index whose cumulative
+ * probability interval contains u, found by descending a pairwise sum tree of p_i = |amp_i|^2
+ * (node = left + right, p_i = fma(re, re, im*im) in double; go left while u < base + left,
+ * never into a zero-sum child). u >= the root sum gives 2^n, whose measured bits are all zero,
+ * as in the reference. The index depends only on the state and u, so any implementation of
+ * the same tree reproduces it bit for bit.
+ */
+int qfvm_sample(qfvm_handle h, const double* uniforms, int64_t shots, uint64_t* out_index);
+
+/* Squared norm of the state (sum of |amp|^2). */
+int qfvm_norm2(qfvm_handle h, double* out);
+
+/* Wait for all work queued on the handle's stream. */
+int qfvm_synchronize(qfvm_handle h);
+
+/*
+ * Kernel accounting (no counterpart in the reference, which has only TICK/TOCK prints,
+ * src/qfvm/util.h:16-23). Every kernel launch is counted with its algorithmic bytes (the bytes it
+ * must read and write); with profiling enabled each launch is also bracketed by HIP events on the
+ * handle's stream. qfvm_profile_json writes {"kernel": {"launches": L, "ms": T, "bytes": B}, ...}
+ * into buf (NUL-terminated, at most cap bytes, *len = full length) and, if reset != 0, clears the
+ * counters.
+ */
+int qfvm_set_profiling(qfvm_handle h, int32_t enable);
+int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QFVM_H */

@@ -1,0 +1,509 @@
+// C-ABI of the engine (include/qfvm.h): handle lifecycle, state I/O, program lowering and the
+// mid-circuit runner. Replaces the reference's pybind entry points (src/qfvm/qfvm.cpp:27-288)
+// and the C++ dispatch they drive (src/qfvm/simulator.hpp:12-125).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+using namespace qfvm;
+
+namespace {
+
+thread_local std::string g_err;
+
+const double kPi = 3.14159265358979323846;  // types.hpp:33
+const cd kI(0.0, 1.0);
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return QFVM_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return QFVM_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return QFVM_ERR_RUNTIME;
+    } catch (...) {
+        g_err = "unknown failure";
+        return QFVM_ERR_RUNTIME;
+    }
+}
+
+State* need(qfvm_handle h) {
+    if (!h) throw Error(QFVM_ERR_VALUE, "null handle");
+    State* s = reinterpret_cast<State*>(h);
+    QFVM_HIP(hipSetDevice(s->device));
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// program parsing and lowering to internal gates (the reference's apply_op dispatch,
+// simulator.hpp:30-113, with the named-gate matrices of statevector.hpp:340-437)
+std::vector<int> op_list(const qfvm_program* p, int64_t off, int32_t n) {
+    if (n < 0 || off < 0 || off + n > p->n_idx || (n > 0 && !p->idx)) throw Error(QFVM_ERR_VALUE, "index pool range");
+    return std::vector<int>(p->idx + off, p->idx + off + n);
+}
+
+void check_qubits(const State& s, const std::vector<int>& q) {
+    for (size_t a = 0; a < q.size(); a++) {
+        if (q[a] < 0 || q[a] >= s.n)
+            throw Error(QFVM_ERR_INDEX,
+                        "qubit index " + std::to_string(q[a]) + " out of range for " + std::to_string(s.n) + " qubits");
+        for (size_t b = 0; b < a; b++)
+            if (q[a] == q[b]) throw Error(QFVM_ERR_VALUE, "repeated qubit " + std::to_string(q[a]));
+    }
+}
+
+std::vector<cd> op_matrix(const qfvm_program* p, const qfvm_op& op, size_t dim) {
+    const int64_t need_d = (int64_t)(2 * dim * dim);
+    if (op.m_off < 0 || op.m_off + need_d > p->n_mat || !p->mat) throw Error(QFVM_ERR_VALUE, "matrix pool range");
+    std::vector<cd> m(dim * dim);
+    for (size_t k = 0; k < dim * dim; k++) m[k] = cd(p->mat[op.m_off + 2 * k], p->mat[op.m_off + 2 * k + 1]);
+    return m;
+}
+
+Gate one(int kind, int target, std::vector<int> ctrls, cd a, cd b, cd c = 0.0, cd d = 0.0) {
+    Gate g;
+    g.kind = kind;
+    g.target = target;
+    g.ctrls = std::move(ctrls);
+    g.cmask = 0;
+    for (int q : g.ctrls) g.cmask |= 1ull << q;
+    g.m[0] = a;
+    g.m[1] = b;
+    g.m[2] = c;
+    g.m[3] = d;
+    return g;
+}
+
+void lower_op(const State& s, const qfvm_program* p, const qfvm_op& op, std::vector<Gate>& out) {
+    std::vector<int> q = op_list(p, op.q_off, op.n_qubits);
+    check_qubits(s, q);
+    auto need_q = [&](size_t k) {
+        if (q.size() < k) throw Error(QFVM_ERR_VALUE, "op needs " + std::to_string(k) + " qubits");
+    };
+    const double th = op.param;
+    switch (op.code) {
+        case QFVM_OP_GENERAL:
+        case QFVM_OP_MULTI: {
+            const int nc = op.n_ctrl;
+            const int nt = (int)q.size() - nc;
+            if (nc < 0 || nt < 1) throw Error(QFVM_ERR_VALUE, "Invalid target number");
+            std::vector<int> ctrls(q.begin(), q.begin() + nc);
+            if (nt == 1) {
+                std::vector<cd> m = op_matrix(p, op, 2);
+                out.push_back(one(G_DENSE, q[nc], ctrls, m[0], m[1], m[2], m[3]));
+            } else {
+                if (nt > 16) throw Error(QFVM_ERR_VALUE, "too many targets");
+                Gate g;
+                g.kind = G_MULTI;
+                g.ctrls = ctrls;
+                for (int c : ctrls) g.cmask |= 1ull << c;
+                g.targets.assign(q.begin() + nc, q.end());
+                g.mk = op_matrix(p, op, 1ull << nt);
+                out.push_back(std::move(g));
+            }
+            return;
+        }
+        case QFVM_OP_X: need_q(1); out.push_back(one(G_SWAP, q[0], {}, 0.0, 1.0, 1.0, 0.0)); return;
+        case QFVM_OP_Y: need_q(1); out.push_back(one(G_DENSE, q[0], {}, 0.0, -kI, kI, 0.0)); return;
+        case QFVM_OP_Z: need_q(1); out.push_back(one(G_DIAG, q[0], {}, 1.0, -1.0)); return;
+        case QFVM_OP_H: {
+            need_q(1);
+            const double r = 1. / std::sqrt(2.);
+            out.push_back(one(G_REAL, q[0], {}, r, r, r, -r));
+            return;
+        }
+        case QFVM_OP_S: need_q(1); out.push_back(one(G_DIAG, q[0], {}, 1.0, kI)); return;
+        case QFVM_OP_SDG: need_q(1); out.push_back(one(G_DIAG, q[0], {}, 1.0, -kI)); return;
+        case QFVM_OP_T: need_q(1); out.push_back(one(G_DIAG, q[0], {}, 1.0, std::exp(kI * kPi / 4.))); return;
+        case QFVM_OP_TDG: need_q(1); out.push_back(one(G_DIAG, q[0], {}, 1.0, std::exp(-kI * kPi / 4.))); return;
+        case QFVM_OP_P: need_q(1); out.push_back(one(G_DIAG, q[0], {}, 1.0, std::exp(kI * th))); return;
+        case QFVM_OP_RX: {
+            need_q(1);
+            const cd c = std::cos(th / 2), sn = -kI * std::sin(th / 2);
+            out.push_back(one(G_DENSE, q[0], {}, c, sn, sn, c));
+            return;
+        }
+        case QFVM_OP_RY: {
+            need_q(1);
+            out.push_back(one(G_REAL, q[0], {}, std::cos(th / 2), -std::sin(th / 2), std::sin(th / 2), std::cos(th / 2)));
+            return;
+        }
+        case QFVM_OP_RZ:
+            need_q(1);
+            out.push_back(one(G_DIAG, q[0], {}, std::exp(-kI * th / 2.), std::exp(kI * th / 2.)));
+            return;
+        case QFVM_OP_CX: need_q(2); out.push_back(one(G_SWAP, q[1], {q[0]}, 0.0, 1.0, 1.0, 0.0)); return;
+        case QFVM_OP_CZ: need_q(2); out.push_back(one(G_DIAG, q[1], {q[0]}, 1.0, -1.0)); return;
+        case QFVM_OP_CP: need_q(2); out.push_back(one(G_DIAG, q[1], {q[0]}, 1.0, std::exp(kI * th))); return;
+        case QFVM_OP_CCX: need_q(3); out.push_back(one(G_SWAP, q[2], {q[0], q[1]}, 0.0, 1.0, 1.0, 0.0)); return;
+        case QFVM_OP_RZZ: {  // simulator.hpp:90-94
+            need_q(2);
+            out.push_back(one(G_SWAP, q[1], {q[0]}, 0.0, 1.0, 1.0, 0.0));
+            out.push_back(one(G_DIAG, q[1], {}, std::exp(-kI * th / 2.), std::exp(kI * th / 2.)));
+            out.push_back(one(G_SWAP, q[1], {q[0]}, 0.0, 1.0, 1.0, 0.0));
+            return;
+        }
+        default:
+            throw Error(QFVM_ERR_VALUE, "op code " + std::to_string(op.code) + " is not a unitary gate");
+    }
+}
+
+bool is_gate_code(int c) { return c >= QFVM_OP_GENERAL && c <= QFVM_OP_RZZ; }
+
+void check_program(const qfvm_program* p) {
+    if (!p) throw Error(QFVM_ERR_VALUE, "null program");
+    if (p->n_ops < 0 || (p->n_ops > 0 && !p->ops)) throw Error(QFVM_ERR_VALUE, "bad op array");
+}
+
+// std::discrete_distribution as implemented by libstdc++ (<bits/random.tcc>), which the reference's
+// sample_measure_probs uses (statevector.hpp:1339-1340): normalise by the sum, partial sums, the
+// last set to 1, first index with u <= cp[i].
+size_t discrete_pick(const std::vector<double>& pr, double u) {
+    if (pr.size() < 2) return 0;
+    double sum = 0.0;
+    for (double x : pr) sum += x;
+    std::vector<double> cp(pr.size());
+    double acc = 0.0;
+    for (size_t i = 0; i < pr.size(); i++) {
+        acc += pr[i] / sum;
+        cp[i] = acc;
+    }
+    cp.back() = 1.0;
+    return (size_t)(std::lower_bound(cp.begin(), cp.end(), u) - cp.begin());
+}
+
+struct Runner {
+    State& s;
+    const qfvm_program* p;
+    const double* u;
+    int64_t nu;
+    int64_t ui;
+    int32_t* creg;
+    int32_t ncreg;
+    bool skip_measure;
+    std::vector<Gate> pending;
+
+    double next_uniform() {
+        if (ui >= nu) throw Error(QFVM_ERR_VALUE, "not enough uniforms for measure/reset");
+        return u[ui++];
+    }
+
+    void flush() {
+        if (!pending.empty()) {
+            apply_gates(s, pending);
+            pending.clear();
+        }
+    }
+
+    void measure_like(const std::vector<int>& q, bool reset, const std::vector<int>* c) {
+        flush();
+        std::vector<double> pr;
+        reduce_marginal(s, q, pr);
+        const size_t m = discrete_pick(pr, next_uniform());
+        const double prob = pr[m];
+        if (!(prob > 0.0)) throw Error(QFVM_ERR_RUNTIME, "measured an outcome of zero probability");
+        launch_collapse(s, q, m, 1.0 / std::sqrt(prob));  // update(), statevector.hpp:1191-1199
+        if (reset && m != 0) {
+            if (q.size() == 1) {
+                std::vector<Gate> x{one(G_SWAP, q[0], {}, 0.0, 1.0, 1.0, 0.0)};
+                apply_gates(s, x);
+            } else {
+                launch_swap_states(s, q, 0, m);
+            }
+        }
+        if (c)
+            for (size_t j = 0; j < q.size(); j++) creg[(*c)[j]] = (int32_t)((m >> j) & 1);
+    }
+
+    void exec(int64_t begin, int64_t end) {
+        int64_t k = begin;
+        while (k < end) {
+            const qfvm_op& op = p->ops[k];
+            if (op.code == QFVM_OP_MEASURE) {
+                k++;
+                if (skip_measure) continue;
+                std::vector<int> q = op_list(p, op.q_off, op.n_qubits);
+                std::vector<int> c = op_list(p, op.c_off, op.n_cbits);
+                check_qubits(s, q);
+                if (c.size() < q.size()) throw Error(QFVM_ERR_VALUE, "measure needs one cbit per qubit");
+                for (size_t j = 0; j < q.size(); j++)
+                    if (c[j] < 0 || c[j] >= ncreg) throw Error(QFVM_ERR_INDEX, "cbit out of range");
+                measure_like(q, false, &c);
+            } else if (op.code == QFVM_OP_RESET) {
+                k++;
+                std::vector<int> q = op_list(p, op.q_off, op.n_qubits);
+                check_qubits(s, q);
+                measure_like(q, true, nullptr);
+            } else if (op.code == QFVM_OP_CIF) {
+                std::vector<int> c = op_list(p, op.c_off, op.n_cbits);
+                const int64_t body_end = k + 1 + op.body;
+                if (op.body < 0 || body_end > end) throw Error(QFVM_ERR_VALUE, "cif body range");
+                uint64_t out = 0;
+                for (int cb : c) {
+                    if (cb < 0 || cb >= ncreg) throw Error(QFVM_ERR_INDEX, "cbit out of range");
+                    out = out * 2 + (uint64_t)creg[cb];
+                }
+                if ((int64_t)out == op.cond) exec(k + 1, body_end);
+                k = body_end;
+            } else {
+                lower_op(s, p, op, pending);
+                k++;
+            }
+        }
+    }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+void* State::ensure_scratch(size_t bytes) {
+    if (bytes > scratch_bytes) {
+        if (scratch) {
+            QFVM_HIP(hipStreamSynchronize(stream));
+            QFVM_HIP(hipFree(scratch));
+            scratch = nullptr;
+            scratch_bytes = 0;
+        }
+        QFVM_HIP(hipMalloc(&scratch, bytes));
+        scratch_bytes = bytes;
+    }
+    return scratch;
+}
+
+double* State::ensure_pinned(size_t bytes) {
+    if (bytes > host_pinned_bytes) {
+        if (host_pinned) {
+            QFVM_HIP(hipStreamSynchronize(stream));
+            QFVM_HIP(hipHostFree(host_pinned));
+            host_pinned = nullptr;
+            host_pinned_bytes = 0;
+        }
+        QFVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_pinned), bytes, hipHostMallocDefault));
+        host_pinned_bytes = bytes;
+    }
+    return host_pinned;
+}
+
+static void destroy_state(State* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    s->resolve_events();
+    if (s->psi) hipFree(s->psi);
+    if (s->scratch) hipFree(s->scratch);
+    if (s->host_pinned) hipHostFree(s->host_pinned);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" {
+
+const char* qfvm_last_error(void) { return g_err.c_str(); }
+
+int qfvm_device_count(int* out) {
+    return guarded([&] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) n = 0;
+        *out = n;
+    });
+}
+
+int qfvm_create(int32_t n_qubits, int32_t dtype, qfvm_handle* out) {
+    return guarded([&] {
+        if (!out) throw Error(QFVM_ERR_VALUE, "null output handle");
+        *out = nullptr;
+        if (n_qubits < 0 || n_qubits > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
+        if (dtype != QFVM_C64 && dtype != QFVM_C128) throw Error(QFVM_ERR_VALUE, "dtype must be QFVM_C64 or QFVM_C128");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error(QFVM_ERR_DEVICE, "no HIP device is visible to this process");
+        std::unique_ptr<State, void (*)(State*)> s(new State(), destroy_state);
+        QFVM_HIP(hipGetDevice(&s->device));
+        s->n = n_qubits;
+        s->dtype = dtype;
+        s->size = 1ull << n_qubits;
+        QFVM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        QFVM_HIP(hipMalloc(&s->psi, s->size * s->amp_bytes()));
+        launch_fill_basis(*s, 0);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+        *out = reinterpret_cast<qfvm_handle>(s.release());
+    });
+}
+
+int qfvm_destroy(qfvm_handle h) {
+    return guarded([&] { destroy_state(reinterpret_cast<State*>(h)); });
+}
+
+int qfvm_num_qubits(qfvm_handle h, int32_t* out) {
+    return guarded([&] { *out = need(h)->n; });
+}
+
+int qfvm_dtype_of(qfvm_handle h, int32_t* out) {
+    return guarded([&] { *out = need(h)->dtype; });
+}
+
+int qfvm_init_zero(qfvm_handle h) {
+    return guarded([&] {
+        State* s = need(h);
+        launch_fill_basis(*s, 0);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int qfvm_init_basis(qfvm_handle h, uint64_t index) {
+    return guarded([&] {
+        State* s = need(h);
+        if (index >= s->size) throw Error(QFVM_ERR_INDEX, "basis index out of range");
+        launch_fill_basis(*s, index);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int qfvm_load(qfvm_handle h, const double* host, uint64_t n_amps) {
+    return guarded([&] {
+        State* s = need(h);
+        if (n_amps != s->size) throw Error(QFVM_ERR_VALUE, "state length must be 2^n");
+        if (!host) throw Error(QFVM_ERR_VALUE, "null host buffer");
+        launch_h2d_convert(*s, host, n_amps);
+    });
+}
+
+int qfvm_store(qfvm_handle h, double* host, uint64_t n_amps) {
+    return guarded([&] {
+        State* s = need(h);
+        if (n_amps != s->size) throw Error(QFVM_ERR_VALUE, "state length must be 2^n");
+        if (!host) throw Error(QFVM_ERR_VALUE, "null host buffer");
+        launch_d2h_convert(*s, host, n_amps);
+    });
+}
+
+int qfvm_apply(qfvm_handle h, const qfvm_program* prog) {
+    return guarded([&] {
+        State* s = need(h);
+        check_program(prog);
+        std::vector<Gate> gates;
+        for (int64_t k = 0; k < prog->n_ops; k++) {
+            if (!is_gate_code(prog->ops[k].code))
+                throw Error(QFVM_ERR_VALUE, "qfvm_apply takes unitary ops only (use qfvm_run for measure/reset/cif)");
+            lower_op(*s, prog, prog->ops[k], gates);
+        }
+        apply_gates(*s, gates);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int qfvm_run(qfvm_handle h, const qfvm_program* prog, const double* uniforms, int64_t n_uniforms, int32_t* creg,
+             int32_t n_creg, int32_t skip_measure) {
+    return guarded([&] {
+        State* s = need(h);
+        check_program(prog);
+        if (n_creg < 0 || (n_creg > 0 && !creg)) throw Error(QFVM_ERR_VALUE, "bad classical register");
+        Runner r{*s, prog, uniforms, n_uniforms, 0, creg, n_creg, skip_measure != 0, {}};
+        r.exec(0, prog->n_ops);
+        r.flush();
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off, const int32_t* len,
+                       int64_t n_terms, double* out) {
+    return guarded([&] {
+        State* s = need(h);
+        for (int64_t t = 0; t < n_terms; t++) {
+            uint64_t flip = 0, z = 0;
+            int flip_q = 0, y_num = 0;
+            for (int32_t k = 0; k < len[t]; k++) {
+                const int q = pos[off[t] + k];
+                if (q < 0 || q >= s->n) throw Error(QFVM_ERR_INDEX, "pauli qubit out of range");
+                switch (chars[off[t] + k]) {  // statevector.hpp:1257-1279 (sums, as the reference)
+                    case 'X': flip += 1ull << q; flip_q = std::max(flip_q, q); break;
+                    case 'Y': flip += 1ull << q; flip_q = std::max(flip_q, q); y_num++; z += 1ull << q; break;
+                    case 'Z': z += 1ull << q; break;
+                    default: break;
+                }
+            }
+            out[t] = reduce_pauli(*s, flip, z, flip_q, y_num);
+        }
+    });
+}
+
+int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n) {
+    return guarded([&] {
+        State* s = need(h);
+        if (n != s->size) throw Error(QFVM_ERR_VALUE, "output length must be 2^n");
+        reduce_probabilities(*s, out);
+    });
+}
+
+int qfvm_sample(qfvm_handle h, const double* uniforms, int64_t shots, uint64_t* out_index) {
+    return guarded([&] {
+        State* s = need(h);
+        if (shots < 0) throw Error(QFVM_ERR_VALUE, "negative shot count");
+        if (shots > 0 && (!uniforms || !out_index)) throw Error(QFVM_ERR_VALUE, "null buffer");
+        sample_tree(*s, uniforms, shots, out_index);
+    });
+}
+
+int qfvm_norm2(qfvm_handle h, double* out) {
+    return guarded([&] { *out = reduce_norm2(*need(h)); });
+}
+
+int qfvm_synchronize(qfvm_handle h) {
+    return guarded([&] {
+        State* s = need(h);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+        s->resolve_events();
+    });
+}
+
+int qfvm_set_profiling(qfvm_handle h, int32_t enable) {
+    return guarded([&] {
+        State* s = need(h);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+        s->resolve_events();
+        s->profiling = enable != 0;
+    });
+}
+
+int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32_t reset) {
+    return guarded([&] {
+        State* s = need(h);
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+        s->resolve_events();
+        std::string j = "{";
+        char tmp[256];
+        for (size_t k = 0; k < s->kstats.size(); k++) {
+            const KStat& st = s->kstats[k].second;
+            std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"launches\": %llu, \"ms\": %.6f, \"bytes\": %.0f}",
+                          k ? ", " : "", s->kstats[k].first.c_str(), (unsigned long long)st.launches, st.ms, st.bytes);
+            j += tmp;
+        }
+        j += "}";
+        if (len) *len = (int64_t)j.size();
+        if (buf && cap > 0) {
+            const size_t m = std::min<size_t>((size_t)cap - 1, j.size());
+            std::memcpy(buf, j.data(), m);
+            buf[m] = 0;
+        }
+        if (reset) s->kstats.clear();
+    });
+}
+
+}  // extern "C"

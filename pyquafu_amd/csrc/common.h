@@ -1,0 +1,161 @@
+// Shared definitions of the MI355X statevector engine: error handling, complex arithmetic,
+// the per-handle state and the internal gate form every kernel consumes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qfvm.h"
+
+namespace qfvm {
+
+using cd = std::complex<double>;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define QFVM_HIP(expr)                                                                                  \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::qfvm::Error(e_ == hipErrorOutOfMemory ? QFVM_ERR_MEMORY : QFVM_ERR_DEVICE,          \
+                                std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+#define QFVM_LAUNCHED()                                                                                 \
+    do {                                                                                                \
+        hipError_t e_ = hipGetLastError();                                                              \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::qfvm::Error(QFVM_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------
+// device complex helpers (interleaved re, im)
+template <typename R>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using T = double2;
+};
+template <>
+struct Vec2<float> {
+    using T = float2;
+};
+template <typename R>
+using V2 = typename Vec2<R>::T;
+
+template <typename R>
+__host__ __device__ inline V2<R> mk(R x, R y) {
+    V2<R> v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+
+// a*b
+template <typename R>
+__device__ inline V2<R> cmul(V2<R> a, V2<R> b) {
+    return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a*b + c*d
+template <typename R>
+__device__ inline V2<R> cmul2(V2<R> a, V2<R> b, V2<R> c, V2<R> d) {
+    return mk<R>(a.x * b.x - a.y * b.y + c.x * d.x - c.y * d.y, a.x * b.y + a.y * b.x + c.x * d.y + c.y * d.x);
+}
+
+// ------------------------------------------------------------------------------------------------
+// internal gate form (host): one target, a control mask and a 2x2 in one of four shapes, or a
+// dense k-target matrix.
+enum GateKind : int {
+    G_DENSE = 0,  // complex 2x2
+    G_REAL = 1,   // real 2x2 (H, RY)
+    G_DIAG = 2,   // diag(d0, d1)
+    G_SWAP = 3,   // X
+    G_MULTI = 4   // dense 2^k x 2^k on k targets (row bit j <-> targets[j])
+};
+
+struct Gate {
+    int kind = G_DENSE;
+    int target = 0;                // single-target kinds
+    uint64_t cmask = 0;            // all control bits must be 1
+    std::vector<int> ctrls;        // the controls (for index insertion)
+    cd m[4] = {1.0, 0.0, 0.0, 1.0};  // DENSE / REAL: m00 m01 m10 m11; DIAG: m[0], m[1]
+    std::vector<int> targets;      // G_MULTI
+    std::vector<cd> mk;            // G_MULTI row-major 4^k
+};
+
+// per-kernel statistics (algorithmic bytes = what the kernel must read + write)
+struct KStat {
+    uint64_t launches = 0;
+    double ms = 0.0;      // summed HIP-event durations (profiling mode only)
+    double bytes = 0.0;   // summed algorithmic bytes
+};
+
+// per-handle state
+struct State {
+    int n = 0;
+    int dtype = QFVM_C128;
+    uint64_t size = 1;
+    void* psi = nullptr;        // device amplitudes (double2 / float2)
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // scratch (grown on demand, freed with the state)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double* host_pinned = nullptr;
+    size_t host_pinned_bytes = 0;
+    // counters: always on; event timing only when `profiling`
+    bool profiling = false;
+    std::vector<std::pair<std::string, KStat>> kstats;
+    struct PendingEvent {
+        int stat;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingEvent> pending;
+
+    size_t amp_bytes() const { return dtype == QFVM_C128 ? 16 : 8; }
+    void* ensure_scratch(size_t bytes);
+    double* ensure_pinned(size_t bytes);
+    int stat_index(const char* name);
+    void resolve_events();  // fold finished event pairs into kstats (call after a stream sync)
+};
+
+// Brackets one kernel launch: counts it and, in profiling mode, records HIP events around it on
+// the state's stream.
+struct KernelScope {
+    State& s;
+    int idx;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelScope(State& st, const char* name, double bytes);
+    ~KernelScope();
+};
+
+// ------------------------------------------------------------------------------------------------
+// kernel launchers (defined in the .hip files)
+void launch_gate(State& s, const Gate& g);
+void launch_fill_basis(State& s, uint64_t index);
+void launch_h2d_convert(State& s, const double* host, uint64_t n);  // host c128 -> device
+void launch_d2h_convert(State& s, double* host, uint64_t n);
+double reduce_norm2(State& s);
+void reduce_probabilities(State& s, double* host_out);
+double reduce_pauli(State& s, uint64_t flip_mask, uint64_t z_mask, int flip_q, int y_num);
+void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double>& probs);
+void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale);
+void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b);
+void sample_tree(State& s, const double* host_u, int64_t shots, uint64_t* host_out);
+
+inline unsigned grid_for(uint64_t work, unsigned block, uint64_t cap = (1ull << 20)) {
+    uint64_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace qfvm

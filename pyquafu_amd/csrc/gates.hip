@@ -1,0 +1,339 @@
+// Per-gate kernels: one launch applies one gate to the whole (control-satisfied) state.
+//
+// These are the unfused path and the fallback for gates the fused pass kernel does not take
+// (dense k-target matrices). Work item j in [0, 2^(n - #positions)) is expanded into the pair
+// (i0, i1 = i0 | 2^t) by inserting zero bits at the sorted target/control positions and OR-ing in
+// the control mask -- the index map of the reference's apply_one_targe_gate_* kernels
+// (src/qfvm/statevector.hpp:446-499), evaluated in 64-bit registers instead of through
+// std::function. Algorithmic bytes per launch: 2 * s * 2^(n-c) (read + write of every amplitude
+// in the control-satisfied subspace), s = 16 (complex128) or 8 (complex64); a diagonal with
+// d0 == 1 touches only the half with the target bit set.
+#include <algorithm>
+
+#include "common.h"
+
+namespace qfvm {
+
+struct InsertSpec {
+    int n;
+    int pos[40];
+};
+
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t j, const InsertSpec& sp) {
+#pragma unroll 1
+    for (int k = 0; k < sp.n; k++) {
+        const uint64_t low = (1ull << sp.pos[k]) - 1;
+        j = (j & low) | ((j & ~low) << 1);
+    }
+    return j;
+}
+
+template <typename R>
+struct Coef {
+    R m[8];  // m00.re m00.im m01.re m01.im m10.re m10.im m11.re m11.im
+};
+
+enum { K_DENSE = 0, K_REAL = 1, K_DIAG = 2, K_DIAG1 = 3, K_SWAP = 4 };
+
+template <typename R, int KIND>
+__global__ void __launch_bounds__(256) k_gate1(V2<R>* __restrict__ psi, uint64_t npairs, InsertSpec sp,
+                                               uint64_t cmask, uint64_t off, Coef<R> c) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < npairs; j += stride) {
+        const uint64_t i0 = insert_zeros(j, sp) | cmask;
+        const uint64_t i1 = i0 | off;
+        if constexpr (KIND == K_DIAG1) {
+            // diag(1, d1): only the target-set half changes
+            V2<R> b = psi[i1];
+            psi[i1] = cmul<R>(mk<R>(c.m[6], c.m[7]), b);
+        } else {
+            V2<R> a = psi[i0];
+            V2<R> b = psi[i1];
+            if constexpr (KIND == K_SWAP) {
+                psi[i0] = b;
+                psi[i1] = a;
+            } else if constexpr (KIND == K_DIAG) {
+                psi[i0] = cmul<R>(mk<R>(c.m[0], c.m[1]), a);
+                psi[i1] = cmul<R>(mk<R>(c.m[6], c.m[7]), b);
+            } else if constexpr (KIND == K_REAL) {
+                psi[i0] = mk<R>(c.m[0] * a.x + c.m[2] * b.x, c.m[0] * a.y + c.m[2] * b.y);
+                psi[i1] = mk<R>(c.m[4] * a.x + c.m[6] * b.x, c.m[4] * a.y + c.m[6] * b.y);
+            } else {
+                psi[i0] = cmul2<R>(mk<R>(c.m[0], c.m[1]), a, mk<R>(c.m[2], c.m[3]), b);
+                psi[i1] = cmul2<R>(mk<R>(c.m[4], c.m[5]), a, mk<R>(c.m[6], c.m[7]), b);
+            }
+        }
+    }
+}
+
+// Dense k-target matrix, 2 <= K <= 5: the 2^K amplitudes of one group live in registers.
+template <int K>
+struct TMask {
+    uint64_t m[1 << K];
+};
+
+template <typename R, int K>
+__global__ void __launch_bounds__(256) k_multi_reg(V2<R>* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
+                                                   uint64_t cmask, TMask<K> tm, const V2<R>* __restrict__ mat) {
+    constexpr int D = 1 << K;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ngroups; j += stride) {
+        const uint64_t base = insert_zeros(j, sp) | cmask;
+        V2<R> v[D];
+#pragma unroll
+        for (int m = 0; m < D; m++) v[m] = psi[base | tm.m[m]];
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            R ax = 0, ay = 0;
+#pragma unroll
+            for (int q = 0; q < D; q++) {
+                const V2<R> w = mat[r * D + q];
+                ax += w.x * v[q].x - w.y * v[q].y;
+                ay += w.x * v[q].y + w.y * v[q].x;
+            }
+            psi[base | tm.m[r]] = mk<R>(ax, ay);
+        }
+    }
+}
+
+// Dense k-target matrix, K >= 6: one block per group; the group is staged in LDS.
+struct TPos {
+    int k;
+    int t[16];
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_multi_lds(V2<R>* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
+                                                   uint64_t cmask, TPos tp, const V2<R>* __restrict__ mat) {
+    extern __shared__ unsigned char lds_raw[];
+    V2<R>* v = reinterpret_cast<V2<R>*>(lds_raw);
+    const int D = 1 << tp.k;
+    for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const uint64_t base = insert_zeros(g, sp) | cmask;
+        for (int m = threadIdx.x; m < D; m += blockDim.x) {
+            uint64_t off = 0;
+            for (int b = 0; b < tp.k; b++) off |= (uint64_t)((m >> b) & 1) << tp.t[b];
+            v[m] = psi[base | off];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < D; r += blockDim.x) {
+            R ax = 0, ay = 0;
+            const V2<R>* row = mat + (uint64_t)r * D;
+            for (int q = 0; q < D; q++) {
+                const V2<R> w = row[q];
+                ax += w.x * v[q].x - w.y * v[q].y;
+                ay += w.x * v[q].y + w.y * v[q].x;
+            }
+            uint64_t off = 0;
+            for (int b = 0; b < tp.k; b++) off |= (uint64_t)((r >> b) & 1) << tp.t[b];
+            psi[base | off] = mk<R>(ax, ay);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_basis_c128(double2* psi, uint64_t size, uint64_t index) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride)
+        psi[i] = make_double2(i == index ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void k_fill_basis_c64(float2* psi, uint64_t size, uint64_t index) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride)
+        psi[i] = make_float2(i == index ? 1.0f : 0.0f, 0.0f);
+}
+
+__global__ void k_widen(const float2* __restrict__ src, double2* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = make_double2((double)src[i].x, (double)src[i].y);
+}
+
+__global__ void k_narrow(const double2* __restrict__ src, float2* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+}
+
+// ------------------------------------------------------------------------------------------------
+static InsertSpec make_spec(std::vector<int> positions) {
+    std::sort(positions.begin(), positions.end());
+    InsertSpec sp{};
+    if (positions.size() > 40) throw Error(QFVM_ERR_VALUE, "too many gate positions");
+    sp.n = (int)positions.size();
+    for (size_t k = 0; k < positions.size(); k++) sp.pos[k] = positions[k];
+    return sp;
+}
+
+template <typename R>
+static Coef<R> make_coef(const Gate& g) {
+    Coef<R> c{};
+    for (int k = 0; k < 4; k++) {
+        c.m[2 * k] = (R)g.m[k].real();
+        c.m[2 * k + 1] = (R)g.m[k].imag();
+    }
+    if (g.kind == G_DIAG) {  // diag stored in m[0], m[1] -> positions 00 and 11
+        c.m[0] = (R)g.m[0].real();
+        c.m[1] = (R)g.m[0].imag();
+        c.m[6] = (R)g.m[1].real();
+        c.m[7] = (R)g.m[1].imag();
+        c.m[2] = c.m[3] = c.m[4] = c.m[5] = 0;
+    }
+    return c;
+}
+
+template <typename R>
+static void gate1_typed(State& s, const Gate& g) {
+    std::vector<int> pos = g.ctrls;
+    pos.push_back(g.target);
+    const InsertSpec sp = make_spec(pos);
+    const uint64_t npairs = s.size >> pos.size();
+    const uint64_t off = 1ull << g.target;
+    const Coef<R> c = make_coef<R>(g);
+    V2<R>* psi = reinterpret_cast<V2<R>*>(s.psi);
+    const unsigned block = 256;
+    const unsigned grid = grid_for(npairs, block);
+    const bool phase_only = g.kind == G_DIAG && g.m[0] == cd(1.0, 0.0);
+    static const char* names[] = {"gate1_dense", "gate1_real", "gate1_diag", "gate1_swap"};
+    KernelScope ks(s, phase_only ? "gate1_phase" : names[g.kind],
+                   (double)npairs * (phase_only ? 2.0 : 4.0) * sizeof(V2<R>));
+    switch (g.kind) {
+        case G_SWAP:
+            hipLaunchKernelGGL((k_gate1<R, K_SWAP>), dim3(grid), dim3(block), 0, s.stream, psi, npairs, sp,
+                               g.cmask, off, c);
+            break;
+        case G_DIAG:
+            if (phase_only)
+                hipLaunchKernelGGL((k_gate1<R, K_DIAG1>), dim3(grid), dim3(block), 0, s.stream, psi, npairs, sp,
+                                   g.cmask, off, c);
+            else
+                hipLaunchKernelGGL((k_gate1<R, K_DIAG>), dim3(grid), dim3(block), 0, s.stream, psi, npairs, sp,
+                                   g.cmask, off, c);
+            break;
+        case G_REAL:
+            hipLaunchKernelGGL((k_gate1<R, K_REAL>), dim3(grid), dim3(block), 0, s.stream, psi, npairs, sp,
+                               g.cmask, off, c);
+            break;
+        default:
+            hipLaunchKernelGGL((k_gate1<R, K_DENSE>), dim3(grid), dim3(block), 0, s.stream, psi, npairs, sp,
+                               g.cmask, off, c);
+            break;
+    }
+    QFVM_LAUNCHED();
+}
+
+template <typename R, int K>
+static void multi_reg_launch(State& s, const Gate& g, const InsertSpec& sp, uint64_t ngroups, const V2<R>* dmat) {
+    TMask<K> tm{};
+    for (int m = 0; m < (1 << K); m++) {
+        uint64_t o = 0;
+        for (int b = 0; b < K; b++)
+            if ((m >> b) & 1) o |= 1ull << g.targets[b];
+        tm.m[m] = o;
+    }
+    const unsigned block = 256;
+    KernelScope ks(s, "multi_reg", (double)ngroups * (1 << K) * 2.0 * sizeof(V2<R>));
+    hipLaunchKernelGGL((k_multi_reg<R, K>), dim3(grid_for(ngroups, block)), dim3(block), 0, s.stream,
+                       reinterpret_cast<V2<R>*>(s.psi), ngroups, sp, g.cmask, tm, dmat);
+    QFVM_LAUNCHED();
+}
+
+template <typename R>
+static void multi_typed(State& s, const Gate& g) {
+    const int k = (int)g.targets.size();
+    const uint64_t dim = 1ull << k;
+    const size_t lds = dim * sizeof(V2<R>);
+    if (k > 16 || lds > 65536) throw Error(QFVM_ERR_VALUE, "dense gate on " + std::to_string(k) + " targets is too large");
+    std::vector<int> pos = g.ctrls;
+    pos.insert(pos.end(), g.targets.begin(), g.targets.end());
+    const InsertSpec sp = make_spec(pos);
+    const uint64_t ngroups = s.size >> pos.size();
+    // matrix -> device (scratch), converted to the state's precision
+    std::vector<V2<R>> hm(dim * dim);
+    for (uint64_t q = 0; q < dim * dim; q++) hm[q] = mk<R>((R)g.mk[q].real(), (R)g.mk[q].imag());
+    V2<R>* dmat = reinterpret_cast<V2<R>*>(s.ensure_scratch(hm.size() * sizeof(V2<R>)));
+    QFVM_HIP(hipMemcpyAsync(dmat, hm.data(), hm.size() * sizeof(V2<R>), hipMemcpyHostToDevice, s.stream));
+    switch (k) {
+        case 2: multi_reg_launch<R, 2>(s, g, sp, ngroups, dmat); break;
+        case 3: multi_reg_launch<R, 3>(s, g, sp, ngroups, dmat); break;
+        case 4: multi_reg_launch<R, 4>(s, g, sp, ngroups, dmat); break;
+        case 5: multi_reg_launch<R, 5>(s, g, sp, ngroups, dmat); break;
+        default: {
+            TPos tp{};
+            tp.k = k;
+            for (int b = 0; b < k; b++) tp.t[b] = g.targets[b];
+            const unsigned grid = (unsigned)std::min<uint64_t>(ngroups, 1u << 16);
+            KernelScope ks(s, "multi_lds", (double)ngroups * dim * 2.0 * sizeof(V2<R>));
+            hipLaunchKernelGGL(k_multi_lds<R>, dim3(grid), dim3(256), lds, s.stream, reinterpret_cast<V2<R>*>(s.psi),
+                               ngroups, sp, g.cmask, tp, dmat);
+            QFVM_LAUNCHED();
+        }
+    }
+    // the host matrix copy must outlive the async copy
+    QFVM_HIP(hipStreamSynchronize(s.stream));
+}
+
+void launch_gate(State& s, const Gate& g) {
+    if (g.kind == G_MULTI) {
+        if (s.dtype == QFVM_C128)
+            multi_typed<double>(s, g);
+        else
+            multi_typed<float>(s, g);
+        return;
+    }
+    if (s.dtype == QFVM_C128)
+        gate1_typed<double>(s, g);
+    else
+        gate1_typed<float>(s, g);
+}
+
+void launch_fill_basis(State& s, uint64_t index) {
+    const unsigned block = 256;
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_fill_basis_c128, dim3(grid_for(s.size, block)), dim3(block), 0, s.stream,
+                           reinterpret_cast<double2*>(s.psi), s.size, index);
+    else
+        hipLaunchKernelGGL(k_fill_basis_c64, dim3(grid_for(s.size, block)), dim3(block), 0, s.stream,
+                           reinterpret_cast<float2*>(s.psi), s.size, index);
+    QFVM_LAUNCHED();
+}
+
+// host complex128 -> device state
+void launch_h2d_convert(State& s, const double* host, uint64_t n) {
+    if (s.dtype == QFVM_C128) {
+        QFVM_HIP(hipMemcpyAsync(s.psi, host, n * 16, hipMemcpyHostToDevice, s.stream));
+        QFVM_HIP(hipStreamSynchronize(s.stream));
+        return;
+    }
+    const uint64_t chunk = 1ull << 22;  // amplitudes per staging round (64 MiB of complex128)
+    double2* stage = reinterpret_cast<double2*>(s.ensure_scratch(std::min(n, chunk) * 16));
+    for (uint64_t b = 0; b < n; b += chunk) {
+        const uint64_t m = std::min(chunk, n - b);
+        QFVM_HIP(hipMemcpyAsync(stage, host + 2 * b, m * 16, hipMemcpyHostToDevice, s.stream));
+        hipLaunchKernelGGL(k_narrow, dim3(grid_for(m, 256)), dim3(256), 0, s.stream, stage,
+                           reinterpret_cast<float2*>(s.psi) + b, m);
+        QFVM_LAUNCHED();
+        QFVM_HIP(hipStreamSynchronize(s.stream));
+    }
+}
+
+void launch_d2h_convert(State& s, double* host, uint64_t n) {
+    if (s.dtype == QFVM_C128) {
+        QFVM_HIP(hipMemcpyAsync(host, s.psi, n * 16, hipMemcpyDeviceToHost, s.stream));
+        QFVM_HIP(hipStreamSynchronize(s.stream));
+        return;
+    }
+    const uint64_t chunk = 1ull << 22;
+    double2* stage = reinterpret_cast<double2*>(s.ensure_scratch(std::min(n, chunk) * 16));
+    for (uint64_t b = 0; b < n; b += chunk) {
+        const uint64_t m = std::min(chunk, n - b);
+        hipLaunchKernelGGL(k_widen, dim3(grid_for(m, 256)), dim3(256), 0, s.stream,
+                           reinterpret_cast<const float2*>(s.psi) + b, stage, m);
+        QFVM_LAUNCHED();
+        QFVM_HIP(hipMemcpyAsync(host + 2 * b, stage, m * 16, hipMemcpyDeviceToHost, s.stream));
+        QFVM_HIP(hipStreamSynchronize(s.stream));
+    }
+}
+
+}  // namespace qfvm

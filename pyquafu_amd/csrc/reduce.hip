@@ -1,0 +1,279 @@
+// Reductions over the state: squared norm, probabilities, Pauli-string expectations, marginal
+// probabilities of measured qubits; and the projective collapse / reset kernels.
+//
+// Sums are deterministic: a fixed grid (independent of the device), a fixed per-thread stride
+// order, a fixed wavefront shuffle tree and a fixed block tree, then one block that adds the
+// per-block partials in index order. Algorithmic bytes: one read of the state (s * 2^n) per
+// reduction.
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+
+namespace qfvm {
+
+constexpr unsigned RED_BLOCK = 256;
+constexpr unsigned RED_GRID = 2048;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Sum of one value per thread over the block; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double wsum[RED_BLOCK / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < (int)(RED_BLOCK / 64); k++) r += wsum[k];
+    }
+    __syncthreads();
+    return r;
+}
+
+template <typename R>
+__device__ __forceinline__ double norm2_of(V2<R> a) {
+    const double x = (double)a.x, y = (double)a.y;
+    return x * x + y * y;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_norm2(const V2<R>* __restrict__ psi, uint64_t size, double* partial) {
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) acc += norm2_of<R>(psi[i]);
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// expect_pauli (statevector.hpp:1250-1305): no flips -> sum |psi_j|^2 (-1)^popc(j & z);
+// with flips -> sum over pairs i0 (bit flip_q clear), i1 = i0 ^ flip of 2 Re(psi_i0 conj(psi_i1) i^k),
+// k = 2*popc(i0 & z) + #Y (mod 4).
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_pauli(const V2<R>* __restrict__ psi, uint64_t work, uint64_t flip,
+                                                      uint64_t zmask, int flip_q, int y_num, double* partial) {
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (flip == 0) {
+        for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < work; j += stride) {
+            const double p = norm2_of<R>(psi[j]);
+            acc += (__popcll(j & zmask) & 1) ? -p : p;
+        }
+    } else {
+        const uint64_t low = (1ull << flip_q) - 1;
+        for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < work; j += stride) {
+            const uint64_t i0 = (j & low) | ((j & ~low) << 1);
+            const uint64_t i1 = i0 ^ flip;
+            const V2<R> a = psi[i0], b = psi[i1];
+            const double re = (double)a.x * b.x + (double)a.y * b.y;  // Re(a conj(b))
+            const double im = (double)a.y * b.x - (double)a.x * b.y;  // Im(a conj(b))
+            const int k = (int)((2 * (__popcll(i0 & zmask) & 1) + y_num) & 3);
+            const double v = (k == 0) ? re : (k == 1) ? -im : (k == 2) ? -re : im;
+            acc += 2.0 * v;
+        }
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_sum_partials(const double* __restrict__ partial, int n, double* out) {
+    // fixed order: thread t adds t, t+256, ...; then the block tree
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partial[k];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) out[0] = acc;
+}
+
+template <typename R>
+__global__ void k_probs(const V2<R>* __restrict__ psi, uint64_t size, double* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) out[i] = norm2_of<R>(psi[i]);
+}
+
+struct QList {
+    int n;
+    int q[64];
+};
+
+__device__ __forceinline__ uint64_t gather_bits(uint64_t j, const QList& ql) {
+    uint64_t m = 0;
+    for (int b = 0; b < ql.n; b++) m |= ((j >> ql.q[b]) & 1ull) << b;
+    return m;
+}
+
+// marginal probabilities of up to 12 qubits: per-block LDS histogram, per-block rows out
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_marginal(const V2<R>* __restrict__ psi, uint64_t size, QList ql,
+                                                         double* __restrict__ rows) {
+    extern __shared__ double hist[];
+    const int bins = 1 << ql.n;
+    for (int b = threadIdx.x; b < bins; b += blockDim.x) hist[b] = 0.0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        const double p = norm2_of<R>(psi[i]);
+        if (p != 0.0) atomicAdd(&hist[gather_bits(i, ql)], p);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < bins; b += blockDim.x) rows[(uint64_t)blockIdx.x * bins + b] = hist[b];
+}
+
+template <typename R>
+__global__ void k_collapse(V2<R>* __restrict__ psi, uint64_t size, QList ql, uint64_t meas, R scale) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        V2<R> a = psi[i];
+        if (gather_bits(i, ql) == meas) {
+            if (scale != (R)1) a = mk<R>(a.x * scale, a.y * scale);
+        } else {
+            a = mk<R>(0, 0);
+        }
+        psi[i] = a;
+    }
+}
+
+template <typename R>
+__global__ void k_swap_states(V2<R>* __restrict__ psi, uint64_t size, uint64_t qmask, uint64_t fmask, uint64_t mmask) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        if ((i & qmask) == fmask) {
+            const uint64_t k = (i & ~qmask) | mmask;
+            const V2<R> a = psi[i];
+            psi[i] = psi[k];
+            psi[k] = a;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+static double finish_sum(State& s, double* partial, unsigned n) {
+    double* out = partial + n;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s.stream, partial, (int)n, out);
+    QFVM_LAUNCHED();
+    double* h = s.ensure_pinned(sizeof(double));
+    QFVM_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    QFVM_HIP(hipStreamSynchronize(s.stream));
+    return h[0];
+}
+
+double reduce_norm2(State& s) {
+    double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
+    const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(s.size, RED_BLOCK));
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_norm2<double>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const double2*>(s.psi), s.size, partial);
+    else
+        hipLaunchKernelGGL(k_norm2<float>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const float2*>(s.psi), s.size, partial);
+    QFVM_LAUNCHED();
+    return finish_sum(s, partial, grid);
+}
+
+double reduce_pauli(State& s, uint64_t flip, uint64_t zmask, int flip_q, int y_num) {
+    double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
+    const uint64_t work = flip ? (s.size >> 1) : s.size;
+    const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(work, RED_BLOCK));
+    KernelScope ks(s, "pauli_expect", (double)s.size * s.amp_bytes());
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_pauli<double>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const double2*>(s.psi), work, flip, zmask, flip_q, y_num, partial);
+    else
+        hipLaunchKernelGGL(k_pauli<float>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const float2*>(s.psi), work, flip, zmask, flip_q, y_num, partial);
+    QFVM_LAUNCHED();
+    return finish_sum(s, partial, grid);
+}
+
+void reduce_probabilities(State& s, double* host_out) {
+    const uint64_t chunk = 1ull << 23;
+    double* stage = reinterpret_cast<double*>(s.ensure_scratch(std::min(s.size, chunk) * sizeof(double)));
+    for (uint64_t b = 0; b < s.size; b += chunk) {
+        const uint64_t m = std::min(chunk, s.size - b);
+        if (s.dtype == QFVM_C128)
+            hipLaunchKernelGGL(k_probs<double>, dim3(grid_for(m, 256)), dim3(256), 0, s.stream,
+                               reinterpret_cast<const double2*>(s.psi) + b, m, stage);
+        else
+            hipLaunchKernelGGL(k_probs<float>, dim3(grid_for(m, 256)), dim3(256), 0, s.stream,
+                               reinterpret_cast<const float2*>(s.psi) + b, m, stage);
+        QFVM_LAUNCHED();
+        QFVM_HIP(hipMemcpyAsync(host_out + b, stage, m * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        QFVM_HIP(hipStreamSynchronize(s.stream));
+    }
+}
+
+static QList make_qlist(const std::vector<int>& q) {
+    QList ql{};
+    if (q.size() > 64) throw Error(QFVM_ERR_VALUE, "too many qubits");
+    ql.n = (int)q.size();
+    for (size_t b = 0; b < q.size(); b++) ql.q[b] = q[b];
+    return ql;
+}
+
+void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double>& probs) {
+    const int k = (int)qbits.size();
+    const uint64_t bins = 1ull << k;
+    probs.assign(bins, 0.0);
+    if (k > 12) {
+        // many measured qubits: full probabilities on the host, then marginalise
+        std::vector<double> p(s.size);
+        reduce_probabilities(s, p.data());
+        for (uint64_t i = 0; i < s.size; i++) {
+            uint64_t m = 0;
+            for (int b = 0; b < k; b++) m |= ((i >> qbits[b]) & 1ull) << b;
+            probs[m] += p[i];
+        }
+        return;
+    }
+    const unsigned grid = std::min<unsigned>(512, grid_for(s.size, RED_BLOCK));
+    double* rows = reinterpret_cast<double*>(s.ensure_scratch((size_t)grid * bins * sizeof(double)));
+    const QList ql = make_qlist(qbits);
+    const size_t lds = bins * sizeof(double);
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_marginal<double>, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+                           reinterpret_cast<const double2*>(s.psi), s.size, ql, rows);
+    else
+        hipLaunchKernelGGL(k_marginal<float>, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+                           reinterpret_cast<const float2*>(s.psi), s.size, ql, rows);
+    QFVM_LAUNCHED();
+    std::vector<double> h((size_t)grid * bins);
+    QFVM_HIP(hipMemcpyAsync(h.data(), rows, h.size() * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    QFVM_HIP(hipStreamSynchronize(s.stream));
+    for (unsigned g = 0; g < grid; g++)
+        for (uint64_t b = 0; b < bins; b++) probs[b] += h[(size_t)g * bins + b];
+}
+
+void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale) {
+    const QList ql = make_qlist(qbits);
+    const unsigned grid = grid_for(s.size, 256);
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_collapse<double>, dim3(grid), dim3(256), 0, s.stream, reinterpret_cast<double2*>(s.psi),
+                           s.size, ql, meas_state, scale);
+    else
+        hipLaunchKernelGGL(k_collapse<float>, dim3(grid), dim3(256), 0, s.stream, reinterpret_cast<float2*>(s.psi),
+                           s.size, ql, meas_state, (float)scale);
+    QFVM_LAUNCHED();
+}
+
+void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b) {
+    uint64_t qmask = 0, fmask = 0, mmask = 0;
+    for (size_t k = 0; k < qbits.size(); k++) {
+        qmask |= 1ull << qbits[k];
+        if ((a >> k) & 1) fmask |= 1ull << qbits[k];
+        if ((b >> k) & 1) mmask |= 1ull << qbits[k];
+    }
+    const unsigned grid = grid_for(s.size, 256);
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_swap_states<double>, dim3(grid), dim3(256), 0, s.stream,
+                           reinterpret_cast<double2*>(s.psi), s.size, qmask, fmask, mmask);
+    else
+        hipLaunchKernelGGL(k_swap_states<float>, dim3(grid), dim3(256), 0, s.stream,
+                           reinterpret_cast<float2*>(s.psi), s.size, qmask, fmask, mmask);
+    QFVM_LAUNCHED();
+}
+
+}  // namespace qfvm

@@ -1,0 +1,37 @@
+"""simulate() (quafu/simulators/__init__.py:25-78) on the MI355X engine."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .program import Program, compile_circuit, compile_ops
+
+
+def simulate(
+    qc,
+    psi: np.ndarray = np.array([]),
+    simulator: str = "statevector",
+    shots: int = 0,
+    hamiltonian=None,
+    use_gpu: bool = False,
+    use_custatevec: bool = False,
+    precision: str = "c128",
+    seed=None,
+):
+    """Simulate a circuit; returns a SimuResult.
+
+    Only the "statevector" simulator is provided (the reference's "clifford" and noisy simulators
+    are other engines). `precision` ("c128" default, or "c64") and `seed` are extensions."""
+    if isinstance(qc, str):
+        raise NotImplementedError("OpenQASM input is not supported by this engine; build a QuantumCircuit")
+    if simulator == "statevector":
+        from .simulator import SVSimulator  # pylint: disable=import-outside-toplevel
+
+        backend = SVSimulator(use_gpu, use_custatevec, precision=precision, seed=seed)
+        return backend.run(qc, psi, shots, hamiltonian)
+    if simulator in ("noisy statevetor", "clifford"):
+        raise NotImplementedError(f"simulator {simulator!r} is not part of this engine")
+    raise ValueError("invalid simulator name")
+
+
+__all__ = ["simulate", "Program", "compile_circuit", "compile_ops"]

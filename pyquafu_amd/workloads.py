@@ -1,0 +1,95 @@
+"""Seeded benchmark and parity workloads (synthetic circuits; numpy PCG64 seeds).
+
+Each builder fixes the order in which it draws random numbers, so the same seed gives the same
+circuit on every machine:
+
+* `random_u3_cx(n, depth, crz_per_layer, seed)`: per layer, angles (theta, phi, lambda) ~ U[0, 2pi)
+  drawn as one (n, 3) array (qubit-major), U3 on every qubit, CX(2i, 2i+1) for all i, then
+  CX(2i+1, 2i+2) for all i; then `crz_per_layer` CRZ gates, each drawn as an ordered pair of distinct
+  qubits (rng.choice(n, 2, replace=False)) followed by theta ~ U[0, 2pi). 20 qubits, depth 20, no CRZ
+  is the CPU-parity workload; 30 qubits, depth 40, 15 CRZ per layer the HBM workload.
+* `hea_ansatz(n, layers, seed)`: per layer RY on every qubit, RZ on every qubit, CZ(i, i+1 mod n);
+  2*n*layers parameters ~ U[-pi, pi] drawn as one array, in gate order.
+* `tfim_hamiltonian(n, h)`: sum_i Z_i Z_{i+1 mod n} + h sum_i X_i.
+* `qft(n, x)`: X on the set bits of x, then the swap-free QFT in textbook order: for target j from
+  n-1 down to 0, H(j) then CP(k -> j, 2 pi / 2^(j-k+1)) for k = j-1 .. 0. Output amplitude of |y> is
+  2^(-n/2) exp(2 pi i x rev(y) / 2^n), rev = bit reversal over n bits.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .algorithms.hamiltonian import Hamiltonian, PauliOp
+from .circuits import QuantumCircuit
+from .elements import CPGate, CRZGate, CZGate, Parameter, RYGate, RZGate, U3Gate
+
+
+def random_u3_cx(n: int, depth: int, crz_per_layer: int = 0, seed: int = 20250001, measure: bool = True):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qc = QuantumCircuit(n)
+    for _ in range(depth):
+        ang = rng.uniform(0.0, 2 * np.pi, size=(n, 3))
+        for q in range(n):
+            qc << U3Gate(q, float(ang[q, 0]), float(ang[q, 1]), float(ang[q, 2]))
+        for i in range(n // 2):
+            qc.cx(2 * i, 2 * i + 1)
+        for i in range((n - 1) // 2):
+            qc.cx(2 * i + 1, 2 * i + 2)
+        for _ in range(crz_per_layer):
+            c, t = rng.choice(n, 2, replace=False)
+            qc << CRZGate(int(c), int(t), float(rng.uniform(0.0, 2 * np.pi)))
+    if measure:
+        qc.measure(list(range(n)))
+    return qc
+
+
+def hea_ansatz(n: int, layers: int, seed: int = 20250003, parametrised: bool = True):
+    """Returns (circuit, values); gate parameters are `Parameter`s when `parametrised`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    values = rng.uniform(-np.pi, np.pi, size=2 * n * layers)
+    qc = QuantumCircuit(n)
+    k = 0
+    for _ in range(layers):
+        for q in range(n):
+            v = Parameter(f"t{k}", values[k]) if parametrised else float(values[k])
+            qc << RYGate(q, v)
+            k += 1
+        for q in range(n):
+            v = Parameter(f"t{k}", values[k]) if parametrised else float(values[k])
+            qc << RZGate(q, v)
+            k += 1
+        for i in range(n):
+            qc << CZGate(i, (i + 1) % n)
+    return qc, values
+
+
+def tfim_hamiltonian(n: int, h: float = 0.7) -> Hamiltonian:
+    terms = [PauliOp(f"Z{i} Z{(i + 1) % n}") for i in range(n)]
+    terms += [PauliOp(f"X{i}", h) for i in range(n)]
+    return Hamiltonian(terms)
+
+
+def qft(n: int, x: int, measure: bool = False):
+    qc = QuantumCircuit(n)
+    for q in range(n):
+        if (x >> q) & 1:
+            qc.x(q)
+    for j in range(n - 1, -1, -1):
+        qc.h(j)
+        for k in range(j - 1, -1, -1):
+            qc << CPGate(k, j, 2 * np.pi / 2 ** (j - k + 1))
+    if measure:
+        qc.measure(list(range(n)))
+    return qc
+
+
+def qft_amplitudes(n: int, x: int) -> np.ndarray:
+    """Closed form of qft(n, x)'s output (float64 / complex128), exact integer phase mod 2^n."""
+    y = np.arange(1 << n, dtype=np.int64)
+    rev = np.zeros_like(y)
+    for b in range(n):
+        rev |= ((y >> b) & 1) << (n - 1 - b)
+    # products wrap mod 2^64, a multiple of 2^n, so the masked result is exact for every n <= 34
+    ph = (np.uint64(x) * rev.astype(np.uint64)) & np.uint64((1 << n) - 1)
+    return np.exp(2j * np.pi * (ph.astype(np.float64) / (1 << n))) / np.sqrt(1 << n)

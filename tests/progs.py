@@ -1,0 +1,87 @@
+"""Random program generators shared by the CPU oracle tests and the GPU parity tests."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from pyquafu_amd.simulators.program import (
+    CCX, CIF, CP, CX, CZ, GENERAL, H, MEASURE, MULTI, P, RESET, RX, RY, RZ, RZZ, S, SDG, T, TDG, X, Y, Z,
+    ProgramBuilder,
+)
+
+NAMED_1Q = [X, Y, Z, H, S, SDG, T, TDG]
+PARAM_1Q = [P, RX, RY, RZ]
+
+
+def random_unitary(d: int, rng) -> np.ndarray:
+    a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    q, r = np.linalg.qr(a)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def random_gate_program(n: int, n_ops: int, rng, max_multi: int = 3, max_ctrl: int = 3):
+    """Every unitary op code, random qubits (controls anywhere), random matrices."""
+    b = ProgramBuilder()
+    for _ in range(n_ops):
+        kind = rng.integers(0, 8)
+        if kind == 0:
+            b.add(int(rng.choice(NAMED_1Q)), [int(rng.integers(n))])
+        elif kind == 1:
+            b.add(int(rng.choice(PARAM_1Q)), [int(rng.integers(n))], param=float(rng.uniform(-7, 7)))
+        elif kind == 2 and n >= 2:
+            q = rng.choice(n, 2, replace=False)
+            code = int(rng.choice([CX, CZ, CP, RZZ]))
+            b.add(code, [int(x) for x in q], n_ctrl=1, param=float(rng.uniform(-7, 7)))
+        elif kind == 3 and n >= 3:
+            q = rng.choice(n, 3, replace=False)
+            b.add(CCX, [int(x) for x in q], n_ctrl=2)
+        elif kind in (4, 5):
+            nc = int(rng.integers(0, min(max_ctrl, n - 1) + 1))
+            q = rng.choice(n, nc + 1, replace=False)
+            b.add(GENERAL, [int(x) for x in q], n_ctrl=nc, matrix=random_unitary(2, rng))
+        elif kind == 6 and n >= 2:
+            k = int(rng.integers(2, min(max_multi, n) + 1))
+            nc = int(rng.integers(0, min(2, n - k) + 1))
+            q = rng.choice(n, k + nc, replace=False)
+            b.add(MULTI, [int(x) for x in q], n_ctrl=nc, matrix=random_unitary(1 << k, rng))
+        else:
+            b.add(GENERAL, [int(rng.integers(n))], matrix=random_unitary(2, rng))
+    return b.program(n_qubits=n, n_cbits=n)
+
+
+def random_classical_program(n: int, rng, n_blocks: int = 4):
+    """Gates interleaved with measure / reset / cif blocks."""
+    b = ProgramBuilder()
+    n_random = 0
+    for _ in range(n_blocks):
+        sub = random_gate_program(n, 6, rng)
+        for row in sub.ops:
+            q = sub.idx[row["q_off"]: row["q_off"] + row["n_qubits"]]
+            m = None
+            if row["m_off"] >= 0:
+                nt = row["n_qubits"] - row["n_ctrl"]
+                d = 1 << nt
+                a = sub.mat[row["m_off"]: row["m_off"] + 2 * d * d]
+                m = (a[0::2] + 1j * a[1::2]).reshape(d, d)
+            b.add(int(row["code"]), list(q), n_ctrl=int(row["n_ctrl"]), matrix=m, param=float(row["param"]))
+        kind = rng.integers(0, 3)
+        k = int(rng.integers(1, min(3, n) + 1))
+        q = [int(x) for x in rng.choice(n, k, replace=False)]
+        if kind == 0:
+            c = [int(x) for x in rng.choice(n, k, replace=False)]
+            b.add(MEASURE, q, cbits=c)
+            n_random += 1
+        elif kind == 1:
+            b.add(RESET, q)
+            n_random += 1
+        else:
+            c = [int(x) for x in rng.choice(n, k, replace=False)]
+            b.add(MEASURE, q, cbits=c)
+            n_random += 1
+            row = b.add(CIF, cbits=c, cond=int(rng.integers(0, 1 << k)))
+            b.add(X, [int(rng.integers(n))])
+            b.add(H, [int(rng.integers(n))])
+            b.set_body(row, 2)
+    p = b.program(n_qubits=n, n_cbits=n)
+    p.n_random = n_random
+    return p

@@ -1,0 +1,287 @@
+"""GPU parity: the HIP engine, called through its C-ABI, against the CPU oracle on the same seeded
+inputs. complex128 amplitudes to 1e-12 max-abs, complex64 to 1e-5; sampled indices bit-exact."""
+
+import numpy as np
+import pytest
+
+import pyquafu_amd as pq
+from oracle.cpu import CpuState, sample_tree_on
+from pyquafu_amd import _native, workloads
+from pyquafu_amd.simulators.program import compile_circuit
+from tests.conftest import rand_state
+from tests.progs import random_classical_program, random_gate_program
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-12, "c64": 1e-5}
+
+
+def gpu_state(n, dtype="c128"):
+    return _native.DeviceState(n, dtype)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 11, 14])
+def test_random_programs(n, dtype):
+    rng = np.random.Generator(np.random.PCG64(100 + n))
+    prog = random_gate_program(n, 80, rng, max_multi=min(5, n))
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    cpu.apply(prog)
+    want = cpu.store()
+    dev = gpu_state(n, dtype)
+    dev.load(psi0)
+    dev.apply(prog)
+    got = dev.store()
+    assert np.abs(got - want).max() < TOL[dtype]
+
+
+def test_large_dense_multi_target():
+    rng = np.random.Generator(np.random.PCG64(7))
+    from tests.progs import random_unitary
+    from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
+
+    n = 12
+    b = ProgramBuilder()
+    b.add(MULTI, [3, 9, 0, 11, 5, 7, 1], n_ctrl=1, matrix=random_unitary(64, rng))
+    b.add(MULTI, [2, 4, 6, 8, 10, 1, 0, 11], n_ctrl=0, matrix=random_unitary(256, rng))
+    prog = b.program(n_qubits=n, n_cbits=n)
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    cpu.apply(prog)
+    dev = gpu_state(n)
+    dev.load(psi0)
+    dev.apply(prog)
+    assert np.abs(dev.store() - cpu.store()).max() < 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_expectations(dtype):
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = 13
+    psi = rand_state(n, rng)
+    terms = [("Z", [0]), ("ZZ", [1, 12]), ("X", [12]), ("XY", [0, 3]), ("YZX", [2, 0, 11]), ("Y", [7]),
+             ("XXYY", [0, 1, 2, 3]), ("IZ", [2, 3]), ("ZZZZZZ", [0, 2, 4, 6, 8, 10])]
+    cpu = CpuState(n)
+    cpu.load(psi)
+    dev = gpu_state(n, dtype)
+    dev.load(psi)
+    assert np.abs(dev.expect(terms) - cpu.expect(terms)).max() < (1e-12 if dtype == "c128" else 1e-5)
+    assert abs(dev.norm2() - 1.0) < (1e-12 if dtype == "c128" else 1e-5)
+    assert np.abs(dev.probabilities() - cpu.probabilities()).max() < (1e-14 if dtype == "c128" else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [0, 1, 4, 5, 6, 12, 17])
+def test_sampler_bit_exact(n, dtype):
+    rng = np.random.Generator(np.random.PCG64(40 + n))
+    psi = rand_state(n, rng)
+    psi[rng.random(psi.size) < 0.3] = 0.0  # zero-probability states must never be drawn
+    if not psi.any():
+        psi[0] = 1.0
+    psi /= np.linalg.norm(psi)
+    dev = gpu_state(n, dtype)
+    dev.load(psi)
+    u = rng.random(100000)
+    u[:4] = [0.0, 1.0 - 2**-53, 1.0, 2.0]
+    got = dev.sample(u)
+    on_gpu_state = dev.store()  # c64 states come back widened exactly
+    want = sample_tree_on(on_gpu_state, u)
+    assert np.array_equal(got, want)
+    nz = np.abs(on_gpu_state) > 0
+    drawn = got[got < (1 << n)].astype(np.int64)
+    assert nz[drawn].all()
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23, 24, 25])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_classical_programs(seed, dtype):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 6
+    prog = random_classical_program(n, rng)
+    u = rng.random(prog.n_random)
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    want_creg = cpu.run(prog, u, n)
+    dev = gpu_state(n, dtype)
+    dev.load(psi0)
+    got_creg = dev.run(prog, u, n)
+    assert np.array_equal(got_creg, want_creg)
+    assert np.abs(dev.store() - cpu.store()).max() < (1e-12 if dtype == "c128" else 1e-5)
+
+
+def test_parity_workload_20q():
+    """20 qubits, complex128, 20 layers of U3 on every qubit + CX brickwork, 10^5 shots."""
+    n = 20
+    qc = workloads.random_u3_cx(n, 20, 0, seed=20250001)
+    prog = compile_circuit(qc)
+    cpu = CpuState(n)
+    cpu.run(prog, [], n, skip_measure=True)
+    want = cpu.store()
+    dev = gpu_state(n)
+    dev.init_zero()
+    dev.run(prog, [], n, True)
+    got = dev.store()
+    err = np.abs(got - want).max()
+    print(f"20q max-abs vs oracle: {err:.3e}")
+    assert err < 1e-12
+    u = np.random.Generator(np.random.PCG64(20250001)).random(100000)
+    idx = dev.sample(u)
+    assert np.array_equal(idx, sample_tree_on(got, u))
+    assert np.array_equal(idx, cpu.sample(u))
+
+
+# ------------------------------------------------------------------ reference known answers
+def sim(qc, **kw):
+    return pq.simulate(qc, seed=7, **kw)
+
+
+def test_reference_basis_known_answers():
+    """tests/quafu/simulator/basis_test.py:128-266 through simulate()."""
+    qc = pq.QuantumCircuit(2)
+    qc.h(0)
+    qc.cx(0, 1)
+    r = sim(qc)
+    assert np.allclose(r.probabilities, [0.5, 0, 0, 0.5]) and r.counts == {}
+    qc = pq.QuantumCircuit(2, 2)
+    qc.x(0)
+    qc.x(1)
+    qc.measure([0, 1])
+    r = sim(qc, shots=1)
+    assert np.allclose(r.probabilities, [0, 0, 0, 1]) and r.counts == {"11": 1}
+    qc = pq.QuantumCircuit(2)
+    qc.x(0)
+    qc.measure([0], [0])
+    qc.x(1)
+    qc.measure([1], [1])
+    r = sim(qc, shots=10)
+    assert np.allclose(r.probabilities, [0, 0, 0, 1]) and r.counts == {"11": 10}
+    qc = pq.QuantumCircuit(2)
+    qc.x(0)
+    qc.measure([0], [0])
+    qc.cx(0, 1)
+    qc.measure([1], [1])
+    r = sim(qc, shots=10)
+    assert r.counts == {"11": 10}
+    qc = pq.QuantumCircuit(4, 4)
+    qc.x(0)
+    qc.x(1)
+    qc.measure([1, 2], [1, 0])
+    qc.measure([3, 0], [2, 3])
+    r = sim(qc, shots=10)
+    assert abs(r.probabilities[5] - 1) < 1e-12 and r.counts == {"0101": 10}
+    qc = pq.QuantumCircuit(3, 2)  # bell_measure_normal with shots=1: collapsed to 00 or 11
+    qc.h(0)
+    qc.cx(0, 1)
+    qc.measure([0, 1])
+    qc.h(2)
+    p = sim(qc, shots=1).probabilities
+    d00 = np.linalg.norm(np.array([1, 0, 0, 0]) - p) ** 2
+    d11 = np.linalg.norm(np.array([0, 0, 0, 1]) - p) ** 2
+    assert np.allclose([d00, d11], [0, 2]) or np.allclose([d00, d11], [2, 0])
+    qc = pq.QuantumCircuit(2, 22)  # after_measure: measure then reset
+    qc.h(0)
+    qc.cx(0, 1)
+    qc.measure([0], [0])
+    qc.measure([1], [1])
+    qc.reset([0, 1])
+    p = sim(qc, shots=10).probabilities
+    d00 = np.linalg.norm(np.array([1, 0, 0, 0]) - p) ** 2
+    d11 = np.linalg.norm(np.array([0, 0, 0, 1]) - p) ** 2
+    assert np.allclose([d00, d11], [0, 2]) or np.allclose([d00, d11], [2, 0])
+
+
+def test_reference_classic_known_answers():
+    """tests/quafu/simulator/classic_test.py:94-185 through simulate()."""
+    qc = pq.QuantumCircuit(2, 2)
+    qc.x(0)
+    qc.measure([0], [0])
+    with qc.cif([0], 1):
+        qc.x(1)
+    qc.measure([1], [1])
+    assert sim(qc, shots=10).counts == {"11": 10}
+    qc = pq.QuantumCircuit(2)
+    qc.measure([0], [0])
+    with qc.cif([0], 1):
+        qc.x(1)
+    qc.measure([1], [1])
+    assert sim(qc, shots=10).counts == {"00": 10}
+    qc = pq.QuantumCircuit(3, 3)
+    qc.x(0)
+    qc.cx(0, 1)
+    qc.measure([0, 1], [0, 1])
+    with qc.cif([0, 1], 3):
+        qc.x(2)
+    qc.measure([2], [2])
+    r = sim(qc, shots=10)
+    assert abs(r.probabilities[7] - 1) < 1e-12 and r.counts == {"111": 10}
+    qc = pq.QuantumCircuit(3)
+    qc.x(0)
+    qc.cx(0, 1)
+    qc.measure([0, 1], [0, 1])
+    for cond in (0, 1, 2, 4):
+        with qc.cif([0, 1], cond):
+            qc.x(2)
+    qc.measure([2], [2])
+    r = sim(qc, shots=10)
+    assert abs(r.probabilities[6] - 1) < 1e-12 and r.counts == {"110": 10}
+    qc = pq.QuantumCircuit(2)
+    qc.x(0)
+    qc.measure([0], [0])
+    qc.reset([0])
+    qc.measure([0], [1])
+    r = sim(qc, shots=10)
+    assert abs(r.probabilities[0] - 1) < 1e-12 and r.counts == {"10": 10}
+    psi = np.zeros(4, dtype=np.complex128)
+    psi[0] = 1.0
+    r = sim(qc, shots=10, psi=psi)
+    assert abs(r.probabilities[0] - 1) < 1e-12 and r.counts == {"10": 10}
+    qc = pq.QuantumCircuit(4)
+    qc.x(0)
+    qc.cx(0, 1)
+    qc.measure([0, 1], [0, 1])
+    qc.reset([0, 1])
+    qc.measure([0, 1], [2, 3])
+    r = sim(qc, shots=10)
+    assert abs(r.probabilities[0] - 1) < 1e-12 and r.counts == {"1100": 10}
+
+
+def test_estimator_reference_value():
+    """estimator_test.py:47-62 + varational_test.py:194-198: <Z0Z1+Z3Z4+Z2Z3+Z0Z4> = 1.0."""
+    from pyquafu_amd.algorithms.hamiltonian import Hamiltonian, PauliOp
+
+    circ = pq.QuantumCircuit(5)
+    for i in range(5):
+        if i % 2 == 0:
+            circ.h(i)
+    circ.cnot(0, 1)
+    circ.measure(list(range(5)))
+    ham = Hamiltonian([PauliOp("Z0 Z1"), PauliOp("Z3 Z4"), PauliOp("Z2 Z3"), PauliOp("Z0 Z4")])
+    assert abs(sum(pq.simulate(circ, hamiltonian=ham)["pauli_expects"]) - 1.0) < 1e-12
+
+
+def test_input_state_adopted_in_place():
+    qc = pq.QuantumCircuit(2)
+    qc.x(1)
+    psi = np.zeros(4, dtype=np.complex128)
+    psi[0] = 1.0
+    out = pq.simulate(qc, psi=psi)["statevector"]
+    assert out is psi and psi[2] == 1.0
+
+
+def test_errors_are_reported():
+    qc = pq.QuantumCircuit(3, 0)
+    qc.x(0)
+    with pytest.raises(RuntimeError):
+        pq.simulate(qc)
+    dev = gpu_state(3)
+    from pyquafu_amd.simulators.program import X, ProgramBuilder
+
+    b = ProgramBuilder()
+    b.add(X, [5])
+    with pytest.raises(_native.EngineError) as e:
+        dev.apply(b.program(n_qubits=3))
+    assert e.value.code == _native.ERR_INDEX
