@@ -85,6 +85,10 @@ def main(argv=None) -> int:
     ap.add_argument("--crz", type=int, default=15)
     ap.add_argument("--dtype", default="c128", choices=["c128", "c64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-fusion", action="store_true", help="one kernel per gate (the unfused path)")
+    ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--low-bits", type=int, default=0)
+    ap.add_argument("--max-groups", type=int, default=0)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the statevector "
@@ -101,6 +105,10 @@ def main(argv=None) -> int:
     print(f"[bench] {n}q {args.dtype} depth {args.depth}: {prog.n_ops} gates", file=sys.stderr, flush=True)
 
     dev = _native.DeviceState(n, args.dtype)
+    dev.set_option("fusion", 0 if args.no_fusion else 1)
+    for key in ("tile_bits", "low_bits", "max_groups"):
+        if getattr(args, key):
+            dev.set_option(key, getattr(args, key))
     dev.init_zero()
     for k in range(args.warmup):
         dev.apply(prog)

@@ -67,7 +67,25 @@ SIGNATURES = {
     "qfvm_synchronize": (ctypes.c_int, [_vp]),
     "qfvm_set_profiling": (ctypes.c_int, [_vp, _i32]),
     "qfvm_profile_json": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64, ctypes.POINTER(_i64), _i32]),
+    "qfvm_set_option": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64]),
+    "qfvm_plan_json": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, ctypes.POINTER(CProgram), ctypes.c_char_p, _i64,
+                                      ctypes.POINTER(_i64)]),
 }
+
+
+def plan_json(prog: Program, n: int, dtype: str = "c128", tile_bits: int = 0, low_bits: int = 0,
+              max_groups: int = 0) -> dict:
+    """The fused execution plan the engine would run for `prog` (host only; no GPU needed)."""
+    import json  # pylint: disable=import-outside-toplevel
+
+    cp, keep = as_cprogram(prog)
+    ln = ctypes.c_int64(0)
+    args = (int(n), _DTYPES[dtype], int(tile_bits), int(low_bits), int(max_groups), ctypes.byref(cp))
+    check(lib().qfvm_plan_json(*args, None, 0, ctypes.byref(ln)))
+    buf = ctypes.create_string_buffer(ln.value + 1)
+    check(lib().qfvm_plan_json(*args, buf, ln.value + 1, ctypes.byref(ln)))
+    del keep
+    return json.loads(buf.value.decode())
 
 
 def lib():
@@ -218,6 +236,9 @@ class DeviceState:
 
     def synchronize(self) -> None:
         check(lib().qfvm_synchronize(self._h))
+
+    def set_option(self, key: str, value: int) -> None:
+        check(lib().qfvm_set_option(self._h, key.encode(), int(value)))
 
     def set_profiling(self, enable: bool) -> None:
         check(lib().qfvm_set_profiling(self._h, int(bool(enable))))

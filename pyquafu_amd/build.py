@@ -23,8 +23,12 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}", "-Wall", "-
 # Device code: gfx950 only; packed-FP32 instructions disabled (they have produced wrong results on
 # these machines); hipcc's host pass warns that it does not know the feature, which is harmless.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# The fused kernel indexes its 16-amplitude register array with compile-time constants in every
+# (gate kind, target bit) case; SimplifyCFG's sinking of identical-looking cases would merge them
+# into one block with a runtime index and push the array to scratch memory.
+DEVICE += ["-mllvm", "-simplifycfg-sink-common=false"]
 
-SOURCES = ["gates.hip", "reduce.hip", "sample.hip", "plan.cpp", "capi.cpp"]
+SOURCES = ["gates.hip", "reduce.hip", "sample.hip", "fused.hip", "plan.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fused.h"
 #include "plan.h"
 
 using namespace qfvm;
@@ -300,12 +301,28 @@ double* State::ensure_pinned(size_t bytes) {
     return host_pinned;
 }
 
+void* State::ensure_ops(size_t bytes) {
+    if (bytes > ops_bytes) {
+        if (ops_buf) {
+            QFVM_HIP(hipStreamSynchronize(stream));
+            QFVM_HIP(hipFree(ops_buf));
+            ops_buf = nullptr;
+            ops_bytes = 0;
+        }
+        const size_t want = std::max<size_t>(bytes, 1 << 20);
+        QFVM_HIP(hipMalloc(&ops_buf, want));
+        ops_bytes = want;
+    }
+    return ops_buf;
+}
+
 static void destroy_state(State* s) {
     if (!s) return;
     hipSetDevice(s->device);
     if (s->stream) hipStreamSynchronize(s->stream);
     s->resolve_events();
     if (s->psi) hipFree(s->psi);
+    if (s->ops_buf) hipFree(s->ops_buf);
     if (s->scratch) hipFree(s->scratch);
     if (s->host_pinned) hipHostFree(s->host_pinned);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -470,6 +487,98 @@ int qfvm_synchronize(qfvm_handle h) {
         State* s = need(h);
         QFVM_HIP(hipStreamSynchronize(s->stream));
         s->resolve_events();
+    });
+}
+
+int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
+    return guarded([&] {
+        State* s = need(h);
+        if (!key) throw Error(QFVM_ERR_VALUE, "null option key");
+        const std::string k(key);
+        if (k == "fusion") {
+            s->fusion = value != 0;
+        } else if (k == "tile_bits") {
+            if (value != 0 && (value < 4 || value > 14)) throw Error(QFVM_ERR_VALUE, "tile_bits must be 0 or 4..14");
+            if (value != 0 && ((1ll << value) * (int64_t)s->amp_bytes() > 160 * 1024))
+                throw Error(QFVM_ERR_VALUE, "tile of 2^tile_bits amplitudes does not fit in 160 KiB of LDS");
+            s->tile_bits = (int)value;
+        } else if (k == "low_bits") {
+            if (value < 0 || value > 14) throw Error(QFVM_ERR_VALUE, "low_bits must be in 0..14");
+            s->low_bits = (int)value;
+        } else if (k == "max_groups") {
+            if (value < 1 || value > 24) throw Error(QFVM_ERR_VALUE, "max_groups must be in 1..24");
+            s->max_groups = (int)value;
+        } else {
+            throw Error(QFVM_ERR_VALUE, "unknown option '" + k + "'");
+        }
+    });
+}
+
+int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
+                   const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
+    return guarded([&] {
+        if (n < 0 || n > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
+        check_program(prog);
+        State tmp;
+        tmp.n = n;
+        tmp.dtype = dtype == QFVM_C64 ? QFVM_C64 : QFVM_C128;
+        tmp.size = 1ull << n;
+        std::vector<Gate> gates;
+        for (int64_t k = 0; k < prog->n_ops; k++) {
+            if (!is_gate_code(prog->ops[k].code)) throw Error(QFVM_ERR_VALUE, "unitary ops only");
+            lower_op(tmp, prog, prog->ops[k], gates);
+        }
+        FuseConfig cfg;
+        cfg.tile_bits = tile_bits > 0 ? tile_bits : (tmp.dtype == QFVM_C128 ? 12 : 13);
+        cfg.low_bits = low_bits > 0 ? low_bits : 6;
+        cfg.max_groups = max_groups > 0 ? max_groups : 12;
+        std::vector<PlanItem> plan = plan_gates(gates, n, cfg);
+        std::string j = "{\"items\": [";
+        char t[512];
+        auto cplx = [&](const cd& z) {
+            std::snprintf(t, sizeof(t), "[%.17g, %.17g]", z.real(), z.imag());
+            return std::string(t);
+        };
+        for (size_t i = 0; i < plan.size(); i++) {
+            const PlanItem& it = plan[i];
+            j += i ? ", " : "";
+            if (it.fused) {
+                j += "{\"fused\": true, \"Q\": [";
+                for (size_t k = 0; k < it.pass.Q.size(); k++) j += (k ? ", " : "") + std::to_string(it.pass.Q[k]);
+                j += "], \"groups\": [";
+                for (size_t g = 0; g < it.pass.groups.size(); g++) {
+                    const FGroup& G = it.pass.groups[g];
+                    j += g ? ", " : "";
+                    std::snprintf(t, sizeof(t), "{\"S\": [%d, %d, %d, %d], \"ops\": [", G.S[0], G.S[1], G.S[2], G.S[3]);
+                    j += t;
+                    for (size_t o = 0; o < G.ops.size(); o++) {
+                        const FOpHost& op = G.ops[o];
+                        std::snprintf(t, sizeof(t), "%s{\"kind\": %d, \"g\": %d, \"cmg\": %u, \"cme\": %llu, \"m\": [", o ? ", " : "",
+                                      op.kind, op.g, op.cm_group, (unsigned long long)op.cm_ext);
+                        j += t;
+                        for (int k = 0; k < 4; k++) j += (k ? ", " : "") + cplx(op.m[k]);
+                        j += "]}";
+                    }
+                    j += "]}";
+                }
+                j += "]}";
+            } else {
+                const Gate& g = it.gate;
+                std::snprintf(t, sizeof(t), "{\"fused\": false, \"kind\": %d, \"target\": %d, \"ctrls\": [", g.kind, g.target);
+                j += t;
+                for (size_t k = 0; k < g.ctrls.size(); k++) j += (k ? ", " : "") + std::to_string(g.ctrls[k]);
+                j += "], \"targets\": [";
+                for (size_t k = 0; k < g.targets.size(); k++) j += (k ? ", " : "") + std::to_string(g.targets[k]);
+                j += "]}";
+            }
+        }
+        j += "]}";
+        if (len) *len = (int64_t)j.size();
+        if (buf && cap > 0) {
+            const size_t m = std::min<size_t>((size_t)cap - 1, j.size());
+            std::memcpy(buf, j.data(), m);
+            buf[m] = 0;
+        }
     });
 }
 

@@ -111,6 +111,13 @@ struct State {
     size_t scratch_bytes = 0;
     double* host_pinned = nullptr;
     size_t host_pinned_bytes = 0;
+    void* ops_buf = nullptr;  // fused-pass op arrays
+    size_t ops_bytes = 0;
+    // execution options (qfvm_set_option); tile_bits / low_bits of 0 pick the defaults
+    int fusion = 1;
+    int tile_bits = 0;
+    int low_bits = 0;
+    int max_groups = 12;
     // counters: always on; event timing only when `profiling`
     bool profiling = false;
     std::vector<std::pair<std::string, KStat>> kstats;
@@ -123,6 +130,7 @@ struct State {
     size_t amp_bytes() const { return dtype == QFVM_C128 ? 16 : 8; }
     void* ensure_scratch(size_t bytes);
     double* ensure_pinned(size_t bytes);
+    void* ensure_ops(size_t bytes);
     int stat_index(const char* name);
     void resolve_events();  // fold finished event pairs into kstats (call after a stream sync)
 };

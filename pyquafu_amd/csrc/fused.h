@@ -1,0 +1,74 @@
+// Fused multi-gate passes: host planner data and the launcher of the LDS-tiled pass kernel.
+//
+// A pass streams the state through LDS once. The state is cut into tiles of 2^b amplitudes: the
+// tile's qubits Q are the L lowest qubits (contiguous in memory, for coalesced loads) plus b - L
+// chosen higher qubits; the other n - b qubits number the tiles. Inside a tile the gates run in
+// "groups": a group names 4 tile qubits; each thread gathers the 16 amplitudes that differ only in
+// those 4 bits from LDS into registers, applies the group's gates there, and writes them back.
+// Gate targets must lie in the group; controls and diagonal gates may sit on any qubit (in or out
+// of the tile), because their effect only depends on index bits each thread knows.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+namespace qfvm {
+
+enum FKind : int32_t {
+    F_DENSE = 0,  // complex 2x2 on group bit g
+    F_REAL = 1,   // real 2x2 on group bit g
+    F_DIAG = 2,   // diag(d0, d1) on group bit g
+    F_SWAP = 3,   // X on group bit g
+    F_PHASE = 4,  // diag(d0, d1) on a qubit outside the group: g = that qubit's global position
+    F_FLUSH = 5   // apply the pending X relabeling of the group bits in g (a 4-bit mask) to the data
+};
+
+// X gates without a control among the group bits move no data: each lane keeps a 4-bit relabeling
+// px (register member P holds logical member P ^ px) that is applied when the row is written back
+// to LDS. A later gate whose target bit is pending uses swapped coefficients (flag FL_TARGET_PX); a
+// gate with a control on a pending bit is preceded by F_FLUSH, which moves the data.
+enum FFlags : uint32_t { FL_TARGET_PX = 1 };
+
+constexpr int FUSE_GROUP_BITS = 4;
+constexpr int FUSE_MAX_GROUPS = 24;
+constexpr int FUSE_MAX_TILE_BITS = 14;
+
+// one gate inside a group (host form; converted to the kernel's precision at upload)
+struct FOpHost {
+    int32_t kind;
+    int32_t g;
+    uint32_t cm_group;  // control bits among the 4 group bits
+    uint32_t flags;     // FFlags
+    uint64_t cm_ext;    // control qubits outside the group (global positions)
+    cd m[4];            // DENSE/REAL: m00 m01 m10 m11; DIAG/PHASE: d0 = m[0], d1 = m[3]
+};
+
+struct FGroup {
+    int S[FUSE_GROUP_BITS];  // global qubits of the group (ascending)
+    std::vector<FOpHost> ops;
+};
+
+struct FPass {
+    std::vector<int> Q;          // tile qubits, ascending; Q[0..L-1] = 0..L-1
+    std::vector<FGroup> groups;
+};
+
+struct FuseConfig {
+    int tile_bits;   // b
+    int low_bits;    // L
+    int max_groups;
+};
+
+// One item of an execution plan: a fused pass or a single gate for the per-gate kernels.
+struct PlanItem {
+    bool fused;
+    FPass pass;
+    Gate gate;
+};
+
+std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg);
+// Runs a plan: fused passes on the LDS-tiled kernel, single gates on the per-gate kernels.
+void execute_plan(State& s, const std::vector<PlanItem>& plan);
+
+}  // namespace qfvm

@@ -1,0 +1,393 @@
+// LDS-tiled fused pass kernel (see fused.h for the tiling).
+//
+// One workgroup (2^(b-4) threads, at most 512: one 16-amplitude row each) owns one tile of 2^b
+// amplitudes:
+//   1. load the tile from HBM into LDS (lanes walk the L low qubits: runs of 2^L contiguous
+//      amplitudes, coalesced);
+//   2. for each group: every thread gathers the 16 amplitudes of one "row" (indices that differ only
+//      in the group's 4 bits) into registers, applies the group's gates with compile-time register
+//      indices, and scatters them back; a barrier separates groups;
+//   3. store the tile back.
+// Algorithmic bytes per launch: 2 * s * 2^n (each amplitude read and written once), s = 16 / 8.
+// LDS addresses are XOR-swizzled (bits 0-2 ^= bits 4-6) so that a row's 16-amplitude stride and the
+// load loop's unit stride both spread across banks.
+#include <algorithm>
+#include <cstring>
+
+#include "fused.h"
+
+namespace qfvm {
+
+template <typename R>
+struct FOpDev {
+    int32_t kind;
+    int32_t g;
+    uint32_t cm_group;
+    uint32_t flags;
+    uint64_t cm_ext;
+    R m[8];
+};
+
+struct PassDesc {
+    int n, b, L;
+    int q[FUSE_MAX_TILE_BITS];  // global qubit of local bit k (ascending; q[k] = k for k < L)
+    int ngroups;
+    int gbits[FUSE_MAX_GROUPS][FUSE_GROUP_BITS];  // local bits of each group, ascending
+    int gop[FUSE_MAX_GROUPS + 1];                  // op ranges in the op array
+};
+
+__device__ __forceinline__ int swz(int l) { return l ^ ((l >> 4) & 7); }
+
+// c ? a : b on the two components (a ternary on the struct itself is lowered through memory)
+template <typename R>
+__device__ __forceinline__ V2<R> csel(bool c, V2<R> a, V2<R> b) {
+    return mk<R>(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+__device__ __forceinline__ int ins_bit(int x, int p) {
+    const int low = (1 << p) - 1;
+    return (x & low) | ((x & ~low) << 1);
+}
+
+// Register indices must be compile-time constants (a runtime index sends the array to scratch), so
+// the 16 members of a row are walked with folds over integer sequences, never with loops. Every
+// branch below is uniform across the wavefront (gate kind, target bit and control masks are the
+// same for all lanes); per-lane facts (controls outside the group, the relabeling px) only select
+// coefficients.
+template <typename R, int KIND, int G, int M, bool CTRL>
+__device__ __forceinline__ void pair_step(V2<R> (&v)[16], int cmg, V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11) {
+    if constexpr ((M >> G) & 1) {
+        return;
+    } else {
+        if constexpr (CTRL) {
+            if ((M & cmg) != cmg) return;
+        }
+        constexpr int M1 = M | (1 << G);
+        const V2<R> a = v[M], b = v[M1];
+        if constexpr (KIND == F_SWAP) {
+            v[M] = b;
+            v[M1] = a;
+        } else if constexpr (KIND == F_DIAG) {
+            v[M] = cmul<R>(m00, a);
+            v[M1] = cmul<R>(m11, b);
+        } else if constexpr (KIND == F_REAL) {
+            v[M] = mk<R>(m00.x * a.x + m01.x * b.x, m00.x * a.y + m01.x * b.y);
+            v[M1] = mk<R>(m10.x * a.x + m11.x * b.x, m10.x * a.y + m11.x * b.y);
+        } else {
+            v[M] = cmul2<R>(m00, a, m01, b);
+            v[M1] = cmul2<R>(m10, a, m11, b);
+        }
+    }
+}
+
+template <typename R, int KIND, int G, int... Ms>
+__device__ __forceinline__ void pairs_seq(V2<R> (&v)[16], int cmg, V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11,
+                                          std::integer_sequence<int, Ms...>) {
+    if (cmg == 0)  // no control among the group bits: no per-pair tests
+        (pair_step<R, KIND, G, Ms, false>(v, cmg, m00, m01, m10, m11), ...);
+    else
+        (pair_step<R, KIND, G, Ms, true>(v, cmg, m00, m01, m10, m11), ...);
+}
+
+// 2x2 kinds (dense, real, diagonal) and controlled X with group controls and outside controls.
+template <typename R, int KIND, int G>
+__device__ __forceinline__ void op_2x2(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, int px) {
+    V2<R> m00 = mk<R>(o.m[0], o.m[1]), m01 = mk<R>(o.m[2], o.m[3]);
+    V2<R> m10 = mk<R>(o.m[4], o.m[5]), m11 = mk<R>(o.m[6], o.m[7]);
+    if (o.flags & FL_TARGET_PX) {
+        // register pair (Q0, Q1) holds logical (L1, L0) where bit G of px is set: the matrix acts
+        // as [[m11, m10], [m01, m00]] on the registers
+        const bool sw = (px >> G) & 1;
+        const V2<R> t00 = m00, t01 = m01, t10 = m10, t11 = m11;
+        m00 = csel<R>(sw, t11, t00);
+        m01 = csel<R>(sw, t10, t01);
+        m10 = csel<R>(sw, t01, t10);
+        m11 = csel<R>(sw, t00, t11);
+    }
+    if (o.cm_ext != 0) {  // lanes whose outside controls fail apply the identity
+        const V2<R> one = mk<R>(1, 0), zero = mk<R>(0, 0);
+        m00 = csel<R>(ok, m00, one);
+        m01 = csel<R>(ok, m01, zero);
+        m10 = csel<R>(ok, m10, zero);
+        m11 = csel<R>(ok, m11, one);
+    }
+    pairs_seq<R, KIND, G>(v, (int)o.cm_group, m00, m01, m10, m11, std::make_integer_sequence<int, 16>{});
+}
+
+// Moves the data of pending relabel bit G: pairs swap where bit G of px is set (bitwise blend, so
+// no select of registers is formed).
+template <typename R, int G, int M>
+__device__ __forceinline__ void flush_step(V2<R> (&v)[16], uint64_t msk) {
+    if constexpr (!((M >> G) & 1)) {
+        constexpr int M1 = M | (1 << G);
+        static_assert(sizeof(V2<R>) % 8 == 0 || sizeof(V2<R>) == 8, "");
+        if constexpr (sizeof(R) == 8) {
+            const uint64_t ax = __builtin_bit_cast(uint64_t, v[M].x), ay = __builtin_bit_cast(uint64_t, v[M].y);
+            const uint64_t bx = __builtin_bit_cast(uint64_t, v[M1].x), by = __builtin_bit_cast(uint64_t, v[M1].y);
+            const uint64_t tx = (ax ^ bx) & msk, ty = (ay ^ by) & msk;
+            v[M].x = __builtin_bit_cast(R, ax ^ tx);
+            v[M].y = __builtin_bit_cast(R, ay ^ ty);
+            v[M1].x = __builtin_bit_cast(R, bx ^ tx);
+            v[M1].y = __builtin_bit_cast(R, by ^ ty);
+        } else {
+            const uint32_t m32 = (uint32_t)msk;
+            const uint32_t ax = __builtin_bit_cast(uint32_t, v[M].x), ay = __builtin_bit_cast(uint32_t, v[M].y);
+            const uint32_t bx = __builtin_bit_cast(uint32_t, v[M1].x), by = __builtin_bit_cast(uint32_t, v[M1].y);
+            const uint32_t tx = (ax ^ bx) & m32, ty = (ay ^ by) & m32;
+            v[M].x = __builtin_bit_cast(R, ax ^ tx);
+            v[M].y = __builtin_bit_cast(R, ay ^ ty);
+            v[M1].x = __builtin_bit_cast(R, bx ^ tx);
+            v[M1].y = __builtin_bit_cast(R, by ^ ty);
+        }
+    }
+}
+
+template <typename R, int G, int... Ms>
+__device__ __forceinline__ void flush_bit(V2<R> (&v)[16], int& px, std::integer_sequence<int, Ms...>) {
+    const uint64_t msk = 0ull - (uint64_t)((px >> G) & 1);
+    (flush_step<R, G, Ms>(v, msk), ...);
+    px &= ~(1 << G);
+}
+
+template <typename R, int M>
+__device__ __forceinline__ void phase_step(V2<R> (&v)[16], int cmg, V2<R> d) {
+    if ((M & cmg) == cmg) v[M] = cmul<R>(d, v[M]);
+}
+
+template <typename R, int... Ms>
+__device__ __forceinline__ void phase_seq(V2<R> (&v)[16], int cmg, V2<R> d, std::integer_sequence<int, Ms...>) {
+    (phase_step<R, Ms>(v, cmg, d), ...);
+}
+
+// LDS slot of every member of a row, computed once per group and used by both gather and scatter.
+template <int... Ms>
+__device__ __forceinline__ void row_slots(int (&a)[16], int l0, int s0, int s1, int s2, int s3,
+                                          std::integer_sequence<int, Ms...>) {
+    ((a[Ms] = swz(l0 | ((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2) |
+                  (((Ms >> 3) & 1) << s3))),
+     ...);
+}
+
+template <typename R, int... Ms>
+__device__ __forceinline__ void gather_seq(V2<R> (&v)[16], const V2<R>* tile, const int (&a)[16],
+                                           std::integer_sequence<int, Ms...>) {
+    ((v[Ms] = tile[a[Ms]]), ...);
+}
+
+// register member P goes back to the slot of logical member P ^ px; the swizzle is linear over
+// XOR, so that slot is a[P] ^ swz(offset of px)
+template <typename R, int... Ms>
+__device__ __forceinline__ void scatter_seq(const V2<R> (&v)[16], V2<R>* tile, const int (&a)[16], int xo,
+                                            std::integer_sequence<int, Ms...>) {
+    ((tile[a[Ms] ^ xo] = v[Ms]), ...);
+}
+
+// global offset of local index bits (local bit k -> qubit q[k]); identity below L
+__device__ __forceinline__ uint64_t local_to_global(int l, const PassDesc& pd) {
+    uint64_t g = (uint64_t)(l & ((1 << pd.L) - 1));
+    for (int k = pd.L; k < pd.b; k++)
+        if ((l >> k) & 1) g |= 1ull << pd.q[k];
+    return g;
+}
+
+template <typename R, int KIND>
+__device__ __forceinline__ void op_kind(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, int px) {
+    switch (o.g) {
+        case 0: op_2x2<R, KIND, 0>(v, o, ok, px); break;
+        case 1: op_2x2<R, KIND, 1>(v, o, ok, px); break;
+        case 2: op_2x2<R, KIND, 2>(v, o, ok, px); break;
+        default: op_2x2<R, KIND, 3>(v, o, ok, px); break;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void apply_op(V2<R> (&v)[16], int& px, const FOpDev<R>& o, uint64_t G0) {
+    const bool ok = (G0 & o.cm_ext) == o.cm_ext;
+    switch (o.kind) {
+        case F_DENSE: op_kind<R, F_DENSE>(v, o, ok, px); break;
+        case F_REAL: op_kind<R, F_REAL>(v, o, ok, px); break;
+        case F_DIAG: op_kind<R, F_DIAG>(v, o, ok, px); break;
+        case F_SWAP:
+            if (o.cm_group == 0) {
+                px ^= ok ? (1 << o.g) : 0;  // relabel only
+            } else if (o.cm_ext == 0) {
+                op_kind<R, F_SWAP>(v, o, ok, px);  // uniform register moves
+            } else {
+                op_kind<R, F_DENSE>(v, o, ok, px);  // X as a matrix, identity where outside controls fail
+            }
+            break;
+        case F_FLUSH:
+            if (o.g & 1) flush_bit<R, 0>(v, px, std::make_integer_sequence<int, 16>{});
+            if (o.g & 2) flush_bit<R, 1>(v, px, std::make_integer_sequence<int, 16>{});
+            if (o.g & 4) flush_bit<R, 2>(v, px, std::make_integer_sequence<int, 16>{});
+            if (o.g & 8) flush_bit<R, 3>(v, px, std::make_integer_sequence<int, 16>{});
+            break;
+        default: {  // F_PHASE: one factor for the whole row, chosen by an index bit outside the group
+            V2<R> d = csel<R>((G0 >> o.g) & 1ull, mk<R>(o.m[6], o.m[7]), mk<R>(o.m[0], o.m[1]));
+            d = csel<R>(ok, d, mk<R>(1, 0));
+            phase_seq<R>(v, (int)o.cm_group, d, std::make_integer_sequence<int, 16>{});
+        }
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, const PassDesc pd,
+                                                       const FOpDev<R>* __restrict__ ops) {
+    extern __shared__ unsigned char lds_raw[];
+    V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
+    const int b = pd.b;
+    const int T = 1 << b;
+    uint64_t base = blockIdx.x;
+    for (int k = 0; k < b; k++) {
+        const uint64_t low = (1ull << pd.q[k]) - 1;
+        base = (base & low) | ((base & ~low) << 1);
+    }
+    // element l = l_hi + tid: the thread's bits (below log2 NT) and the loop's bits deposit into
+    // disjoint global bits, so the loop part is uniform and stays in scalar registers
+    const int tid = threadIdx.x;
+    const int NT = blockDim.x;  // 64 .. 512, a power of two
+    const uint64_t g_tid = base | local_to_global(tid, pd);
+    if (sizeof(R) == 8 && T >= NT && NT >= 128) {
+        // direct HBM -> LDS loads (gfx950 global_load_lds_dwordx4): LDS slot s = lh + tid receives
+        // element swz(s) = lh + swz(tid) (the swizzle only mixes bits 0-6, and lh is a multiple of
+        // NT >= 128), so no VGPRs hold the tile in flight
+        const uint64_t g_sw = base | local_to_global(swz(tid), pd);
+        const int wave = tid >> 6;
+#pragma unroll 16
+        for (int lh = 0; lh < T; lh += NT) {
+            const void* src = (const void*)(psi + (g_sw | local_to_global(lh, pd)));
+            auto* dst = (__attribute__((address_space(3))) void*)(tile + lh + wave * 64);
+            if constexpr (sizeof(R) == 8) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+#pragma unroll 16
+        for (int lh = 0; lh < T; lh += NT)
+            if (lh + tid < T) tile[swz(lh + tid)] = psi[g_tid | local_to_global(lh, pd)];
+    }
+    __syncthreads();
+
+    const int rows = T >> 4;
+    for (int gi = 0; gi < pd.ngroups; gi++) {
+        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];
+        const int o_begin = pd.gop[gi], o_end = pd.gop[gi + 1];
+        for (int r = tid; r < rows; r += NT) {
+            const int l0 = ins_bit(ins_bit(ins_bit(ins_bit(r, s0), s1), s2), s3);
+            int a[16];
+            row_slots(a, l0, s0, s1, s2, s3, std::make_integer_sequence<int, 16>{});
+            V2<R> v[16];
+            gather_seq<R>(v, tile, a, std::make_integer_sequence<int, 16>{});
+            const uint64_t G0 = base | local_to_global(l0, pd);
+            int px = 0;
+            for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, px, ops[oi], G0);
+            const int xo = swz(((px & 1) << s0) | (((px >> 1) & 1) << s1) | (((px >> 2) & 1) << s2) |
+                               (((px >> 3) & 1) << s3));
+            scatter_seq<R>(v, tile, a, xo, std::make_integer_sequence<int, 16>{});
+        }
+        __syncthreads();
+    }
+
+#pragma unroll 4
+    for (int lh = 0; lh < T; lh += NT) {
+        if (lh + tid < T) psi[g_tid | local_to_global(lh, pd)] = tile[swz(lh + tid)];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+template <typename R>
+static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R>>& h) {
+    const int b = (int)p.Q.size();
+    int L = 0;
+    while (L < b && p.Q[L] == L) L++;
+    PassDesc pd{};
+    pd.n = s.n;
+    pd.b = b;
+    pd.L = L;
+    for (int k = 0; k < b; k++) pd.q[k] = p.Q[k];
+    pd.ngroups = (int)p.groups.size();
+    if (pd.ngroups > FUSE_MAX_GROUPS || b > FUSE_MAX_TILE_BITS || b < FUSE_GROUP_BITS || b > s.n)
+        throw Error(QFVM_ERR_RUNTIME, "fused pass exceeds kernel limits");
+    int local_of[64];
+    for (int k = 0; k < 64; k++) local_of[k] = -1;
+    for (int k = 0; k < b; k++) local_of[p.Q[k]] = k;
+    const int first = (int)h.size();
+    for (int gi = 0; gi < pd.ngroups; gi++) {
+        const FGroup& G = p.groups[gi];
+        for (int j = 0; j < FUSE_GROUP_BITS; j++) {
+            if (local_of[G.S[j]] < 0) throw Error(QFVM_ERR_RUNTIME, "group qubit outside the tile");
+            pd.gbits[gi][j] = local_of[G.S[j]];
+        }
+        pd.gop[gi] = (int)h.size() - first;
+        for (const FOpHost& o : G.ops) {
+            FOpDev<R> d{};
+            d.kind = o.kind;
+            d.g = o.g;
+            d.cm_group = o.cm_group;
+            d.flags = o.flags;
+            d.cm_ext = o.cm_ext;
+            for (int k = 0; k < 4; k++) {
+                d.m[2 * k] = (R)o.m[k].real();
+                d.m[2 * k + 1] = (R)o.m[k].imag();
+            }
+            h.push_back(d);
+        }
+    }
+    pd.gop[pd.ngroups] = (int)h.size() - first;
+    return pd;
+}
+
+template <typename R>
+static void execute_typed(State& s, const std::vector<PlanItem>& plan) {
+    // encode every pass's ops into one array, upload once, then launch the items in order
+    std::vector<FOpDev<R>> h;
+    std::vector<PassDesc> descs;
+    std::vector<size_t> first;
+    for (const PlanItem& it : plan) {
+        if (!it.fused) continue;
+        first.push_back(h.size());
+        descs.push_back(encode_pass<R>(s, it.pass, h));
+    }
+    FOpDev<R>* dops = nullptr;
+    if (!h.empty()) {
+        const size_t bytes = h.size() * sizeof(FOpDev<R>);
+        QFVM_HIP(hipStreamSynchronize(s.stream));  // the pinned stage may still feed an earlier copy
+        dops = reinterpret_cast<FOpDev<R>*>(s.ensure_ops(bytes));
+        void* stage = s.ensure_pinned(bytes);
+        std::memcpy(stage, h.data(), bytes);
+        QFVM_HIP(hipMemcpyAsync(dops, stage, bytes, hipMemcpyHostToDevice, s.stream));
+    }
+    size_t k = 0;
+    for (const PlanItem& it : plan) {
+        if (!it.fused) {
+            launch_gate(s, it.gate);
+            continue;
+        }
+        const PassDesc& pd = descs[k];
+        const FOpDev<R>* ops = dops + first[k];
+        k++;
+        const size_t lds = ((size_t)1 << pd.b) * sizeof(V2<R>);
+        if (lds > 65536) {
+            static bool attr_set = false;
+            if (!attr_set) {
+                QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<R>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                attr_set = true;
+            }
+        }
+        const uint64_t tiles = 1ull << (s.n - pd.b);
+        KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
+        const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
+        hipLaunchKernelGGL(k_fused_pass<R>, dim3((unsigned)tiles), dim3(threads), lds, s.stream,
+                           reinterpret_cast<V2<R>*>(s.psi), pd, ops);
+        QFVM_LAUNCHED();
+    }
+}
+
+void execute_plan(State& s, const std::vector<PlanItem>& plan) {
+    if (s.dtype == QFVM_C128)
+        execute_typed<double>(s, plan);
+    else
+        execute_typed<float>(s, plan);
+}
+
+}  // namespace qfvm

@@ -1,16 +1,205 @@
-// Gate-run execution. This first version launches one pair-update kernel per gate.
+// Gate-run execution: the fusion planner and kernel accounting.
+//
+// The planner packs a gate list into fused passes (fused.h). It walks the gates in order and takes
+// every gate it can move into the current pass without changing the result:
+//   * a gate's non-diagonal qubits N (the target of a dense / real / X gate) must fit in the tile,
+//     which grows from the L low qubits up to b qubits;
+//   * a gate that cannot be taken is skipped, and every later gate that does not commute with a
+//     skipped one is skipped too. Commutation is checked on qubit roles: two gates commute when every
+//     qubit they share is only a control or a diagonal target for both;
+//   * inside the pass the gate joins the latest group whose 4 qubits can hold N (a group may grow
+//     until it has 4 qubits) and may move back past later groups it commutes with.
+// Diagonal gates (RZ, CZ, CP, controlled phases, diagonal 2x2 matrices) have N empty, so they never
+// widen a tile. Dense gates on two or more targets run on the per-gate kernels.
+#include <algorithm>
+
+#include "fused.h"
 #include "plan.h"
 
 namespace qfvm {
 
-void apply_gates(State& s, const std::vector<Gate>& gates) {
-    for (const Gate& g : gates) launch_gate(s, g);
+namespace {
+
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+struct GInfo {
+    uint64_t N = 0, D = 0;
+    bool diag = false, multi = false;
+    cd d0, d1;
+};
+
+GInfo info_of(const Gate& g) {
+    GInfo r;
+    r.D = g.cmask;
+    if (g.kind == G_MULTI) {
+        r.multi = true;
+        for (int t : g.targets) r.N |= 1ull << t;
+        return r;
+    }
+    const bool dense_diag = g.kind == G_DENSE && g.m[1] == cd(0.0, 0.0) && g.m[2] == cd(0.0, 0.0);
+    if (g.kind == G_DIAG || dense_diag) {
+        r.diag = true;
+        r.D |= 1ull << g.target;
+        r.d0 = g.m[0];
+        r.d1 = g.kind == G_DIAG ? g.m[1] : g.m[3];
+    } else {
+        r.N = 1ull << g.target;
+    }
+    return r;
 }
 
-}  // namespace qfvm
+struct GroupB {
+    uint64_t S = 0, N = 0, A = 0;
+    std::vector<int> gates;
+};
 
-namespace qfvm {
+FOpHost convert(const Gate& g, const GInfo& gi, const int S[4]) {
+    FOpHost o{};
+    auto idx_in = [&](int q) {
+        for (int j = 0; j < FUSE_GROUP_BITS; j++)
+            if (S[j] == q) return j;
+        return -1;
+    };
+    for (int c : g.ctrls) {
+        const int j = idx_in(c);
+        if (j >= 0)
+            o.cm_group |= 1u << j;
+        else
+            o.cm_ext |= 1ull << c;
+    }
+    if (gi.diag) {
+        const int j = idx_in(g.target);
+        o.kind = j >= 0 ? F_DIAG : F_PHASE;
+        o.g = j >= 0 ? j : g.target;
+        o.m[0] = gi.d0;
+        o.m[1] = 0.0;
+        o.m[2] = 0.0;
+        o.m[3] = gi.d1;
+        return o;
+    }
+    o.g = idx_in(g.target);
+    o.kind = g.kind == G_SWAP ? F_SWAP : g.kind == G_REAL ? F_REAL : F_DENSE;
+    for (int k = 0; k < 4; k++) o.m[k] = g.m[k];
+    return o;
+}
 
+}  // namespace
+
+std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg) {
+    const int b = std::min(cfg.tile_bits, n);
+    const int L = std::min(cfg.low_bits, b);
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    std::vector<GInfo> info(gates.size());
+    for (size_t i = 0; i < gates.size(); i++) info[i] = info_of(gates[i]);
+    std::vector<char> done(gates.size(), 0);
+    std::vector<PlanItem> plan;
+    size_t first = 0;
+    while (true) {
+        while (first < gates.size() && done[first]) first++;
+        if (first >= gates.size()) break;
+        if (info[first].multi) {
+            plan.push_back(PlanItem{false, FPass{}, gates[first]});
+            done[first] = 1;
+            continue;
+        }
+        uint64_t Q = L >= 64 ? ~0ull : ((1ull << L) - 1);
+        uint64_t blockN = 0, blockA = 0;
+        std::vector<GroupB> groups;
+        for (size_t i = first; i < gates.size(); i++) {
+            if (done[i]) continue;
+            const GInfo& g = info[i];
+            const uint64_t A = g.N | g.D;
+            bool take = !((g.N & blockA) || (g.D & blockN)) && !g.multi;
+            uint64_t newQ = Q | g.N;
+            if (take && popc(newQ) > b) take = false;
+            int place = -1;
+            if (take) {
+                for (int j = (int)groups.size() - 1; j >= 0; j--) {
+                    const GroupB& G = groups[j];
+                    if ((G.S & g.N) == g.N || popc(G.S | g.N) <= FUSE_GROUP_BITS) {
+                        place = j;
+                        break;
+                    }
+                    if ((g.N & G.A) || (g.D & G.N)) break;  // cannot move before group j
+                }
+                if (place < 0) {
+                    if ((int)groups.size() < cfg.max_groups) {
+                        groups.emplace_back();
+                        place = (int)groups.size() - 1;
+                    } else {
+                        take = false;
+                    }
+                }
+            }
+            if (!take) {
+                blockN |= g.N;
+                blockA |= A;
+                if ((blockN & all) == all) break;  // every later gate touches a blocked qubit
+                continue;
+            }
+            GroupB& G = groups[place];
+            G.S |= g.N;
+            G.N |= g.N;
+            G.A |= A;
+            G.gates.push_back((int)i);
+            Q = newQ;
+            done[i] = 1;
+        }
+        // finalize: pad the tile to b qubits and every group to 4 tile qubits
+        for (int q = 0; q < n && popc(Q) < b; q++) Q |= 1ull << q;
+        PlanItem it{true, FPass{}, Gate{}};
+        for (int q = 0; q < n; q++)
+            if ((Q >> q) & 1) it.pass.Q.push_back(q);
+        for (GroupB& G : groups) {
+            for (int q : it.pass.Q) {
+                if (popc(G.S) >= FUSE_GROUP_BITS) break;
+                G.S |= 1ull << q;
+            }
+            FGroup fg;
+            int j = 0;
+            for (int q = 0; q < n; q++)
+                if ((G.S >> q) & 1) fg.S[j++] = q;
+            // X gates without group controls become relabelings (fused.h); track the pending bits
+            uint32_t pending = 0;
+            for (int gi : G.gates) {
+                FOpHost o = convert(gates[gi], info[gi], fg.S);
+                const uint32_t need = o.cm_group & pending;
+                if (need) {
+                    FOpHost f{};
+                    f.kind = F_FLUSH;
+                    f.g = (int)need;
+                    fg.ops.push_back(f);
+                    pending &= ~need;
+                }
+                if (o.kind == F_SWAP) {
+                    if (o.cm_group == 0) pending |= 1u << o.g;
+                } else if (o.kind != F_PHASE && ((pending >> o.g) & 1)) {
+                    o.flags |= FL_TARGET_PX;
+                }
+                fg.ops.push_back(o);
+            }
+            it.pass.groups.push_back(std::move(fg));
+        }
+        plan.push_back(std::move(it));
+    }
+    return plan;
+}
+
+void apply_gates(State& s, const std::vector<Gate>& gates) {
+    if (gates.empty()) return;
+    if (!s.fusion || s.n < 6) {
+        for (const Gate& g : gates) launch_gate(s, g);
+        return;
+    }
+    FuseConfig cfg;
+    cfg.tile_bits = s.tile_bits > 0 ? s.tile_bits : (s.dtype == QFVM_C128 ? 12 : 13);
+    cfg.tile_bits = std::max(FUSE_GROUP_BITS, std::min(cfg.tile_bits, FUSE_MAX_TILE_BITS));
+    cfg.low_bits = s.low_bits > 0 ? s.low_bits : 6;
+    cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
+    execute_plan(s, plan_gates(gates, s.n, cfg));
+}
+
+// ------------------------------------------------------------------------------------------------
 int State::stat_index(const char* name) {
     for (size_t k = 0; k < kstats.size(); k++)
         if (kstats[k].first == name) return (int)k;

@@ -285,3 +285,50 @@ def test_errors_are_reported():
     with pytest.raises(_native.EngineError) as e:
         dev.apply(b.program(n_qubits=3))
     assert e.value.code == _native.ERR_INDEX
+
+
+# ------------------------------------------------------------------ fused passes
+@pytest.mark.parametrize("dtype,tile,low,groups", [("c128", 12, 6, 12), ("c128", 12, 3, 4), ("c128", 13, 4, 24),
+                                                   ("c128", 8, 2, 2), ("c64", 13, 6, 12), ("c64", 14, 4, 24)])
+def test_fused_matches_oracle(dtype, tile, low, groups):
+    rng = np.random.Generator(np.random.PCG64(tile * 100 + low))
+    n = 17
+    prog = random_gate_program(n, 400, rng, max_multi=3)
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    cpu.apply(prog)
+    dev = gpu_state(n, dtype)
+    for k, v in (("tile_bits", tile), ("low_bits", low), ("max_groups", groups)):
+        dev.set_option(k, v)
+    dev.load(psi0)
+    dev.apply(prog)
+    assert np.abs(dev.store() - cpu.store()).max() < TOL[dtype]
+    assert dev.profile()["fused_pass"]["launches"] > 0
+
+
+def test_fused_equals_unfused_at_24_qubits():
+    """The two GPU paths on the HBM workload's structure at 24 qubits, and the oracle."""
+    n = 24
+    prog = compile_circuit(workloads.random_u3_cx(n, 6, 15, seed=31, measure=False))
+    a = gpu_state(n)
+    a.apply(prog)
+    fa = a.store()
+    a.close()
+    b = gpu_state(n)
+    b.set_option("fusion", 0)
+    b.apply(prog)
+    fb = b.store()
+    b.close()
+    assert np.abs(fa - fb).max() < 1e-12
+    cpu = CpuState(n)
+    cpu.apply(prog)
+    assert np.abs(fa - cpu.store()).max() < 1e-12
+
+
+def test_option_validation():
+    dev = gpu_state(8)
+    with pytest.raises(_native.EngineError):
+        dev.set_option("tile_bits", 14)  # 2^14 complex128 amplitudes exceed the LDS
+    with pytest.raises(_native.EngineError):
+        dev.set_option("no_such_option", 1)

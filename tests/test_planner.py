@@ -1,0 +1,114 @@
+"""The fusion planner on the CPU: the plan the engine would execute (qfvm_plan_json) is replayed
+op by op in NumPy and must reproduce the oracle's state. This checks the planner's reordering,
+grouping and control / phase conversion without a GPU; the kernel that executes plans is checked
+against the oracle by the GPU parity tests."""
+
+import numpy as np
+import pytest
+
+from oracle import dense
+from oracle.cpu import CpuState
+from pyquafu_amd import _native, workloads
+from pyquafu_amd.simulators.program import compile_circuit
+from tests.conftest import rand_state
+from tests.progs import random_gate_program
+
+F_DENSE, F_REAL, F_DIAG, F_SWAP, F_PHASE, F_FLUSH = range(6)
+G_MULTI = 4
+
+
+def replay(plan, prog, n, psi):
+    """Executes a plan's fused ops as controlled one-qubit gates on global qubits, in plan order.
+    Non-fused items are the original MULTI ops, taken from the program in order."""
+    psi = np.array(psi, dtype=complex)
+    multi_rows = [r for r in prog.ops if r["code"] == 1]
+    mi = 0
+    idx = np.arange(1 << n, dtype=np.int64)
+    for it in plan["items"]:
+        if not it["fused"]:
+            row = multi_rows[mi]
+            mi += 1
+            q = prog.idx[row["q_off"]: row["q_off"] + row["n_qubits"]].tolist()
+            nc = int(row["n_ctrl"])
+            d = 1 << (len(q) - nc)
+            a = prog.mat[row["m_off"]: row["m_off"] + 2 * d * d]
+            psi = dense.apply_unitary(psi, n, q[:nc], q[nc:], (a[0::2] + 1j * a[1::2]).reshape(d, d))
+            continue
+        Q = it["Q"]
+        assert Q == sorted(Q) and len(set(Q)) == len(Q)
+        for G in it["groups"]:
+            S = G["S"]
+            assert len(set(S)) == 4 and set(S) <= set(Q)
+            for op in G["ops"]:
+                if op["kind"] == F_FLUSH:  # moves data between registers only; no logical effect
+                    assert 0 < op["g"] < 16
+                    continue
+                m = [complex(*z) for z in op["m"]]
+                cm = int(op["cme"])
+                for j in range(4):
+                    if (op["cmg"] >> j) & 1:
+                        cm |= 1 << S[j]
+                if op["kind"] == F_PHASE:
+                    q = op["g"]
+                    assert q not in S
+                else:
+                    q = S[op["g"]]
+                assert not (cm >> q) & 1
+                ok = (idx & cm) == cm
+                bit = (idx >> q) & 1
+                if op["kind"] in (F_DIAG, F_PHASE):
+                    f = np.where(bit == 1, m[3], m[0])
+                    psi = np.where(ok, psi * f, psi)
+                    continue
+                i0 = idx[(bit == 0) & ok]
+                i1 = i0 | (1 << q)
+                a, b = psi[i0].copy(), psi[i1].copy()
+                if op["kind"] == F_SWAP:
+                    psi[i0], psi[i1] = b, a
+                else:
+                    if op["kind"] == F_REAL:
+                        m = [z.real for z in m]
+                    psi[i0] = m[0] * a + m[1] * b
+                    psi[i1] = m[2] * a + m[3] * b
+    assert mi == len(multi_rows)
+    return psi
+
+
+def check(prog, n, **kw):
+    plan = _native.plan_json(prog, n, **kw)
+    rng = np.random.Generator(np.random.PCG64(n))
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    cpu.apply(prog)
+    got = replay(plan, prog, n, psi0)
+    assert np.abs(got - cpu.store()).max() < 1e-12
+    return plan
+
+
+@pytest.mark.parametrize("n,tile,low,groups", [(6, 0, 0, 0), (9, 6, 2, 2), (10, 8, 4, 3), (12, 0, 0, 0),
+                                                (13, 7, 3, 12), (14, 12, 6, 24)])
+def test_random_programs_replay(n, tile, low, groups):
+    rng = np.random.Generator(np.random.PCG64(500 + n))
+    prog = random_gate_program(n, 300, rng, max_multi=3)
+    check(prog, n, tile_bits=tile, low_bits=low, max_groups=groups)
+
+
+def test_brickwork_plan_is_compact():
+    n = 20
+    prog = compile_circuit(workloads.random_u3_cx(n, 6, 5, seed=9, measure=False))
+    plan = check(prog, n)
+    passes = [it for it in plan["items"] if it["fused"]]
+    assert all(it["fused"] for it in plan["items"])
+    # 6 layers of a 20-qubit brickwork need at least 6 * 14 / 6 passes with 6 free tile qubits
+    assert len(passes) <= 6 * 4, len(passes)
+    total_ops = sum(1 for it in passes for g in it["groups"] for op in g["ops"] if op["kind"] != F_FLUSH)
+    assert total_ops == prog.n_ops
+
+
+def test_qft_and_ansatz_plans():
+    n = 14
+    prog = compile_circuit(workloads.qft(n, 12345))
+    check(prog, n)
+    qc, _ = workloads.hea_ansatz(n, 3, parametrised=False)
+    check(compile_circuit(qc), n)
