@@ -13,6 +13,11 @@ Printed (one JSON line):
                  step time: comparable with the CPU, may exceed the HBM peak once gates are fused
   cpu_baseline   the CPU oracle (a restatement of qfvm's OpenMP engine, built with its flags) on
                  the first layer of the same circuit, same unit as gate_equiv
+
+roofline.traffic is the HBM bytes per launch of the dominant kernel read from rocprofv3 FETCH_SIZE
+and WRITE_SIZE runs, each scaled by the same counter on a plain streaming kernel of known size
+(`--copy-calibration`); the figure is kept in profiles/<kernel>_<n>q_<dtype>_traffic.txt and is null
+when no such file exists for the workload.
 """
 
 from __future__ import annotations
@@ -75,6 +80,33 @@ def cpu_baseline(prog, n: int, layer_gates: int, amp_bytes: int):
     }
 
 
+def traffic_from_profile(kernel: str, n: int, dtype: str):
+    """HBM bytes per launch recorded by the counter runs for this kernel and workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{n}q_{dtype}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def copy_calibration() -> int:
+    """A plain streaming kernel of known size (torch.neg over 4 GiB of float64: reads 4 GiB and
+    writes 4 GiB per launch), run 3 times, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    nbytes = 1 << 32
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"op": "torch.neg float64", "read_bytes_per_launch": nbytes,
+                                           "write_bytes_per_launch": nbytes, "launches": 3}}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +121,15 @@ def main(argv=None) -> int:
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--max-groups", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only the streaming calibration kernel for the HBM counters")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the statevector "
               "engine holds the whole state on one MI355X; no multi-GPU figure is produced)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration()
 
     from pyquafu_amd import _native, workloads
     from pyquafu_amd.simulators.program import compile_circuit
@@ -169,7 +205,7 @@ def main(argv=None) -> int:
             "peak": PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic_from_profile(dom_name, n, args.dtype),
             "bytes_per_launch": per_launch_bytes,
             "mean_launch_ms": round(per_launch_s * 1e3, 4),
         },
