@@ -153,6 +153,29 @@ int qfvm_run(qfvm_handle h, const qfvm_program* prog, const double* uniforms, in
 int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
                        const int32_t* len, int64_t n_terms, double* out);
 
+/*
+ * <psi|H|psi> for H = sum_t coef_t P_t (coef as re, im pairs; terms laid out as in
+ * qfvm_expect_paulis, no qubit repeated within a term): the real part of the sum that
+ * SVSimulator.run forms from expect_statevec and the coefficients (simulator.py:133-139), taken in
+ * one pass over the state.
+ */
+int qfvm_expect_hamiltonian(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
+                            const int32_t* len, const double* coef, int64_t n_terms, double* out);
+
+/*
+ * Reverse-mode gradient of <psi|H|psi>, psi = U(prog)|state> (grad_adjoint, gradiant.py:144-173,
+ * arXiv:2009.02823). prog holds unitary ops only. For each g < n_grads, op grad_ops[g] must be a
+ * one-target rotation exp(-i theta P / 2) with P = grad_gen[g] ('X', 'Y' or 'Z'; controls allowed);
+ * grads[g] = d<H>/d theta = 2 Re <lambda| dU |phi> with dU = -i/2 P U on the control-satisfied
+ * subspace (grad_gate, gradiant.py:96-141), lambda = U_{>k}^dagger H psi and phi the state before
+ * the op. *expectation (if not null) receives <psi|H|psi>. The sweep keeps a second state on the
+ * device; on return the handle holds the state just before the first differentiated op.
+ */
+int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_ops, const char* grad_gen,
+                 int64_t n_grads, const char* chars, const int32_t* pos, const int64_t* off,
+                 const int32_t* len, const double* coef, int64_t n_terms, double* grads,
+                 double* expectation);
+
 /* |amp|^2 for every basis state (probabilities(), statevector.hpp:1023-1032). */
 int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n);
 

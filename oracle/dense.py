@@ -146,38 +146,32 @@ class DenseRunner:
         a = self.p.mat[op["m_off"]: op["m_off"] + 2 * dim * dim]
         return (a[0::2] + 1j * a[1::2]).reshape(dim, dim)
 
-    def gate(self, op):
+    def gate_parts(self, op):
+        """(controls, targets, target matrix) of one gate op."""
         q = self._q(op)
         c, th = int(op["code"]), float(op["param"])
-        n = self.n
         if c in (GENERAL, MULTI):
             nc = int(op["n_ctrl"])
             nt = len(q) - nc
-            self.psi = apply_unitary(self.psi, n, q[:nc], q[nc:], self._m(op, 1 << nt))
-            return
+            return q[:nc], q[nc:], self._m(op, 1 << nt)
         one = {X: XM, Y: YM, Z: ZM, H: HM, S: SM, SDG: SM.conj().T, T: TM, TDG: TM.conj().T}
         if c in one:
-            self.psi = apply_unitary(self.psi, n, [], [q[0]], one[c])
-        elif c == P:
-            self.psi = apply_unitary(self.psi, n, [], [q[0]], pm(th))
-        elif c == RX:
-            self.psi = apply_unitary(self.psi, n, [], [q[0]], rx(th))
-        elif c == RY:
-            self.psi = apply_unitary(self.psi, n, [], [q[0]], ry(th))
-        elif c == RZ:
-            self.psi = apply_unitary(self.psi, n, [], [q[0]], rz(th))
-        elif c == CX:
-            self.psi = apply_unitary(self.psi, n, [q[0]], [q[1]], XM)
-        elif c == CZ:
-            self.psi = apply_unitary(self.psi, n, [q[0]], [q[1]], ZM)
-        elif c == CP:
-            self.psi = apply_unitary(self.psi, n, [q[0]], [q[1]], pm(th))
-        elif c == CCX:
-            self.psi = apply_unitary(self.psi, n, [q[0], q[1]], [q[2]], XM)
-        elif c == RZZ:
-            self.psi = apply_unitary(self.psi, n, [], [q[0], q[1]], rzz_lsb(th))
-        else:
-            raise ValueError(f"not a gate: {c}")
+            return [], [q[0]], one[c]
+        table = {P: ([], [0], pm), RX: ([], [0], rx), RY: ([], [0], ry), RZ: ([], [0], rz), CP: ([0], [1], pm)}
+        if c in table:
+            cs, ts, f = table[c]
+            return [q[k] for k in cs], [q[k] for k in ts], f(th)
+        fixed = {CX: ([0], [1], XM), CZ: ([0], [1], ZM), CCX: ([0, 1], [2], XM)}
+        if c in fixed:
+            cs, ts, m = fixed[c]
+            return [q[k] for k in cs], [q[k] for k in ts], m
+        if c == RZZ:
+            return [], [q[0], q[1]], rzz_lsb(th)
+        raise ValueError(f"not a gate: {c}")
+
+    def gate(self, op):
+        ctrls, targs, m = self.gate_parts(op)
+        self.psi = apply_unitary(self.psi, self.n, ctrls, targs, m)
 
     def exec(self, begin: int, end: int) -> None:
         k = begin

@@ -61,6 +61,9 @@ SIGNATURES = {
     "qfvm_apply": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram)]),
     "qfvm_run": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram), _vp, _i64, _vp, _i32, _i32]),
     "qfvm_expect_paulis": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "qfvm_expect_hamiltonian": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "qfvm_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram), _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64,
+                                    _vp, _vp]),
     "qfvm_probabilities": (ctypes.c_int, [_vp, _vp, _u64]),
     "qfvm_sample": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "qfvm_norm2": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl)]),
@@ -150,6 +153,16 @@ def pauli_arrays(terms):
     )
 
 
+def _coeff_array(coeffs, n: int) -> np.ndarray:
+    cf = np.asarray(coeffs, dtype=np.complex128).reshape(-1)
+    if cf.size != n:
+        raise ValueError("one coefficient per Pauli term")
+    out = np.zeros(2 * max(n, 1), dtype=np.float64)
+    out[0: 2 * n: 2] = cf.real
+    out[1: 2 * n: 2] = cf.imag
+    return out
+
+
 class DeviceState:
     """One statevector resident in HBM, owned by the engine."""
 
@@ -216,6 +229,32 @@ class DeviceState:
             check(lib().qfvm_expect_paulis(self._h, c.ctypes.data, p.ctypes.data, o.ctypes.data, l_.ctypes.data,
                                            len(o), out.ctypes.data))
         return out
+
+    def expect_hamiltonian(self, terms, coeffs) -> float:
+        """<psi|H|psi> for H = sum_t coeffs[t] * P_t, terms as for `expect`."""
+        c, p, o, l_ = pauli_arrays(terms)
+        cf = _coeff_array(coeffs, len(o))
+        out = ctypes.c_double(0.0)
+        check(lib().qfvm_expect_hamiltonian(self._h, c.ctypes.data, p.ctypes.data, o.ctypes.data, l_.ctypes.data,
+                                            cf.ctypes.data, len(o), ctypes.byref(out)))
+        return out.value
+
+    def adjoint(self, prog: Program, grad_ops, grad_gens: str, terms, coeffs):
+        """Reverse-mode gradient of <H> after `prog` (qfvm_adjoint). Returns (grads, expectation)."""
+        cp, keep = as_cprogram(prog)
+        ops = np.ascontiguousarray(grad_ops, dtype=np.int64)
+        gens = np.frombuffer(grad_gens.encode() + b"\0", dtype=np.uint8)
+        if len(grad_gens) != ops.size:
+            raise ValueError("one generator letter per differentiated op")
+        c, p, o, l_ = pauli_arrays(terms)
+        cf = _coeff_array(coeffs, len(o))
+        grads = np.zeros(max(ops.size, 1), dtype=np.float64)
+        e = ctypes.c_double(0.0)
+        check(lib().qfvm_adjoint(self._h, ctypes.byref(cp), ops.ctypes.data, gens.ctypes.data, ops.size,
+                                 c.ctypes.data, p.ctypes.data, o.ctypes.data, l_.ctypes.data, cf.ctypes.data, len(o),
+                                 grads.ctypes.data, ctypes.byref(e)))
+        del keep
+        return grads[: ops.size], e.value
 
     def probabilities(self) -> np.ndarray:
         out = np.empty(1 << self.n, dtype=np.float64)

@@ -1,0 +1,128 @@
+"""Gradients of <H> for parameterised circuits (quafu/algorithms/gradients/gradiant.py:24-173).
+
+grad_adjoint runs the whole reverse-mode sweep on the device in one engine call (qfvm_adjoint):
+the forward state, lambda = H psi and the reverse sweep over both states with the gradient markers
+fused into the same LDS-tiled passes. The reference instead loops over gates in Python with one
+engine round trip, a state copy and a host inner product per gate (gradiant.py:160-172); the
+values are the same (d<H>/d theta = 2 Re <lambda| dU |phi>, grad_gate's dU = -i/2 P U on the
+control-satisfied subspace, gradiant.py:96-141), and the chain rule through ParameterExpression
+is assembled on the host exactly as assemble_grads does (gradiant.py:24-35).
+
+grad_para_shift and grad_finit_diff keep the reference's algorithms (gradiant.py:38-93): one full
+simulation per shifted value through `backend.run`.
+"""
+
+from __future__ import annotations
+
+from typing import List
+
+import numpy as np
+
+from ... import _native
+from ...elements.parameters import Parameter, ParameterExpression
+from ...elements.quantum_gate import ControlledGate
+from ...exceptions import CircuitError
+from ...simulators.program import compile_gates
+from ...simulators.qfvm import _call
+from ...simulators.simulator import SVSimulator
+
+_GENERATORS = {"RX": "X", "RY": "Y", "RZ": "Z"}
+
+
+def assemble_grads(para_grads, gate_grads) -> List[float]:
+    """Chain rule: d/d var = sum over (gate, parameter slot) uses of d<H>/d slot * d slot/d var."""
+    grads = []
+    for var in para_grads:
+        full = 0.0
+        for (i, j), gp in para_grads[var]:
+            full += gate_grads[i][j] * gp
+        grads.append(full)
+    return grads
+
+
+def _is_symbolic(op) -> bool:
+    return len(op.paras) > 0 and isinstance(op.paras[0], (Parameter, ParameterExpression))
+
+
+def generator_of(op) -> str:
+    """Pauli generator of a differentiable rotation (grad_gate supports RX, RY, RZ and their
+    controlled forms, gradiant.py:96-141; anything else raises NotImplementedError)."""
+    name = op._targ_name if isinstance(op, ControlledGate) else op.name  # pylint: disable=protected-access
+    gen = _GENERATORS.get(str(name).upper())
+    if gen is None or (isinstance(op, ControlledGate) and len(op.targs) != 1):
+        raise NotImplementedError(f"no adjoint gradient for gate {op.name}")
+    return gen
+
+
+def hamiltonian_terms(hamiltonian):
+    """([(paulistr, pos), ...], coefficients) of a Hamiltonian's PauliOps."""
+    paulis = hamiltonian.paulis
+    return [(p.paulistr, list(p.pos)) for p in paulis], [complex(p.coeff) for p in paulis]
+
+
+def adjoint_program(qc):
+    """(program, op rows of the differentiated gates, generator letters, their gate indices)."""
+    gates = qc.gates
+    prog, rows = compile_gates(gates, qc.num)
+    diff = [i for i, op in enumerate(gates) if _is_symbolic(op)]
+    return prog, [rows[i] for i in diff], "".join(generator_of(gates[i]) for i in diff), diff
+
+
+def grad_adjoint(qc, hamiltonian, psi_in=np.array([], dtype=complex), *, precision: str = "c128",
+                 return_expectation: bool = False):
+    """Reverse-mode gradient (arXiv:2009.02823) of <H> with respect to qc.variables' order of
+    `_calc_parameter_grads`. With return_expectation, returns (grads, <H>)."""
+    if not qc._parameter_grads:  # pylint: disable=protected-access
+        qc.get_parameter_grads()
+    para_grads = qc._calc_parameter_grads()  # pylint: disable=protected-access
+    prog, ops, gens, diff = adjoint_program(qc)
+    terms, coeffs = hamiltonian_terms(hamiltonian)
+    dev = _call(_native.pool.get, qc.num, precision)
+    psi_in = np.asarray(psi_in)
+    if psi_in.size:
+        _call(dev.load, psi_in)
+    else:
+        _call(dev.init_zero)
+    values, expectation = _call(dev.adjoint, prog, ops, gens, terms, coeffs)
+    gate_grads = [[] for _ in qc.gates]
+    for i, v in zip(diff, values):
+        gate_grads[i].append(float(v))
+    grads = assemble_grads(para_grads, gate_grads)
+    return (grads, expectation) if return_expectation else grads
+
+
+def grad_para_shift(qc, hamiltonian, backend=None, psi_in=np.array([], dtype=complex)):
+    """Parameter-shift gradients (gradiant.py:38-73): each symbolic gate must be RX, RY or RZ."""
+    backend = SVSimulator() if backend is None else backend
+    if not qc._parameter_grads:  # pylint: disable=protected-access
+        qc.get_parameter_grads()
+    para_grads = qc._calc_parameter_grads()  # pylint: disable=protected-access
+    gate_grads = [[] for _ in qc.gates]
+    for i, op in enumerate(qc.gates):
+        if _is_symbolic(op):
+            if op.name not in ["RX", "RY", "RZ"]:
+                raise CircuitError(
+                    "It seems the circuit can not apply parameter-shift rule to calculate gradient."
+                    " You may need compile the circuit first"
+                )
+            op.paras[0] = op.paras[0] + np.pi / 2
+            res1 = sum(backend.run(qc, hamiltonian=hamiltonian, psi=psi_in)["pauli_expects"])
+            op.paras[0] = op.paras[0] - np.pi
+            res2 = sum(backend.run(qc, hamiltonian=hamiltonian, psi=psi_in)["pauli_expects"])
+            op.paras[0]._undo(2)  # pylint: disable=protected-access
+            gate_grads[i].append((res1 - res2) / 2.0)
+    return assemble_grads(para_grads, gate_grads)
+
+
+def grad_finit_diff(qc, hamiltonian, backend=None, psi_in=np.array([], dtype=complex), h: float = 1e-10):
+    """Central finite differences over qc.variables (gradiant.py:76-93; the step is a keyword here)."""
+    backend = SVSimulator() if backend is None else backend
+    grads = []
+    for v in qc.variables:
+        v.value += h
+        res1 = sum(backend.run(qc, hamiltonian=hamiltonian, psi=psi_in)["pauli_expects"])
+        v.value -= 2 * h
+        res2 = sum(backend.run(qc, hamiltonian=hamiltonian, psi=psi_in)["pauli_expects"])
+        v.value += h
+        grads.append((res1 - res2) / (2 * h))
+    return grads

@@ -167,6 +167,73 @@ void lower_op(const State& s, const qfvm_program* p, const qfvm_op& op, std::vec
 
 bool is_gate_code(int c) { return c >= QFVM_OP_GENERAL && c <= QFVM_OP_RZZ; }
 
+// U^dagger of an internal gate (the reference builds op.dagger() per gate, gradiant.py:163-172)
+Gate dagger(const Gate& g) {
+    Gate d = g;
+    switch (g.kind) {
+        case G_DENSE:
+            d.m[0] = std::conj(g.m[0]);
+            d.m[1] = std::conj(g.m[2]);
+            d.m[2] = std::conj(g.m[1]);
+            d.m[3] = std::conj(g.m[3]);
+            break;
+        case G_REAL:
+            d.m[1] = g.m[2];
+            d.m[2] = g.m[1];
+            break;
+        case G_DIAG:
+            d.m[0] = std::conj(g.m[0]);
+            d.m[1] = std::conj(g.m[1]);
+            break;
+        case G_MULTI: {
+            const size_t dim = (size_t)1 << g.targets.size();
+            for (size_t r = 0; r < dim; r++)
+                for (size_t c = 0; c < dim; c++) d.mk[r * dim + c] = std::conj(g.mk[c * dim + r]);
+            break;
+        }
+        default: break;  // X is its own inverse
+    }
+    return d;
+}
+
+// Pauli-sum observable from the flat term arrays of qfvm_expect_paulis plus complex coefficients;
+// terms grouped by flip mask in ascending order.
+Hamiltonian parse_hamiltonian(const State& s, const char* chars, const int32_t* pos, const int64_t* off,
+                              const int32_t* len, const double* coef, int64_t n_terms) {
+    if (n_terms < 0) throw Error(QFVM_ERR_VALUE, "negative term count");
+    if (n_terms > 0 && (!chars || !pos || !off || !len || !coef)) throw Error(QFVM_ERR_VALUE, "null term array");
+    std::vector<std::pair<uint64_t, HamTerm>> all;
+    for (int64_t t = 0; t < n_terms; t++) {
+        uint64_t flip = 0, zy = 0, seen = 0;
+        int ny = 0;
+        for (int32_t k = 0; k < len[t]; k++) {
+            const int q = pos[off[t] + k];
+            if (q < 0 || q >= s.n) throw Error(QFVM_ERR_INDEX, "pauli qubit out of range");
+            if ((seen >> q) & 1) throw Error(QFVM_ERR_VALUE, "repeated qubit in a Pauli term");
+            seen |= 1ull << q;
+            switch (chars[off[t] + k]) {
+                case 'X': flip |= 1ull << q; break;
+                case 'Y': flip |= 1ull << q; zy |= 1ull << q; ny++; break;
+                case 'Z': zy |= 1ull << q; break;
+                case 'I': break;
+                default: throw Error(QFVM_ERR_VALUE, "Pauli letters are I, X, Y, Z");
+            }
+        }
+        cd c(coef[2 * t], coef[2 * t + 1]);
+        for (int k = 0; k < (ny & 3); k++) c *= kI;
+        all.push_back({flip, HamTerm{zy, c.real(), c.imag()}});
+    }
+    std::stable_sort(all.begin(), all.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    Hamiltonian H;
+    for (const auto& e : all) {
+        if (H.groups.empty() || H.groups.back().flip != e.first)
+            H.groups.push_back(HamGroup{e.first, (int32_t)H.terms.size(), 0});
+        H.terms.push_back(e.second);
+        H.groups.back().count++;
+    }
+    return H;
+}
+
 void check_program(const qfvm_program* p) {
     if (!p) throw Error(QFVM_ERR_VALUE, "null program");
     if (p->n_ops < 0 || (p->n_ops > 0 && !p->ops)) throw Error(QFVM_ERR_VALUE, "bad op array");
@@ -316,12 +383,34 @@ void* State::ensure_ops(size_t bytes) {
     return ops_buf;
 }
 
+void* State::ensure_aux(size_t bytes) {
+    if (bytes > aux_bytes) {
+        if (aux_buf) {
+            QFVM_HIP(hipStreamSynchronize(stream));
+            QFVM_HIP(hipFree(aux_buf));
+            aux_buf = nullptr;
+            aux_bytes = 0;
+        }
+        const size_t want = std::max<size_t>(bytes, 1 << 16);
+        QFVM_HIP(hipMalloc(&aux_buf, want));
+        aux_bytes = want;
+    }
+    return aux_buf;
+}
+
+void* State::ensure_psi2() {
+    if (!psi2) QFVM_HIP(hipMalloc(&psi2, size * amp_bytes()));
+    return psi2;
+}
+
 static void destroy_state(State* s) {
     if (!s) return;
     hipSetDevice(s->device);
     if (s->stream) hipStreamSynchronize(s->stream);
     s->resolve_events();
     if (s->psi) hipFree(s->psi);
+    if (s->psi2) hipFree(s->psi2);
+    if (s->aux_buf) hipFree(s->aux_buf);
     if (s->ops_buf) hipFree(s->ops_buf);
     if (s->scratch) hipFree(s->scratch);
     if (s->host_pinned) hipHostFree(s->host_pinned);
@@ -458,6 +547,90 @@ int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, con
             }
             out[t] = reduce_pauli(*s, flip, z, flip_q, y_num);
         }
+    });
+}
+
+int qfvm_expect_hamiltonian(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
+                            const int32_t* len, const double* coef, int64_t n_terms, double* out) {
+    return guarded([&] {
+        State* s = need(h);
+        if (!out) throw Error(QFVM_ERR_VALUE, "null output");
+        const Hamiltonian H = parse_hamiltonian(*s, chars, pos, off, len, coef, n_terms);
+        *out = H.groups.empty() ? 0.0 : apply_hamiltonian(*s, H, nullptr);
+    });
+}
+
+int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_ops, const char* grad_gen,
+                 int64_t n_grads, const char* chars, const int32_t* pos, const int64_t* off, const int32_t* len,
+                 const double* coef, int64_t n_terms, double* grads, double* expectation) {
+    return guarded([&] {
+        State* s = need(h);
+        check_program(prog);
+        if (n_grads < 0 || (n_grads > 0 && (!grad_ops || !grad_gen || !grads)))
+            throw Error(QFVM_ERR_VALUE, "bad gradient arrays");
+        const Hamiltonian H = parse_hamiltonian(*s, chars, pos, off, len, coef, n_terms);
+        // forward gates, with the range of internal gates each op lowers to
+        std::vector<Gate> fwd;
+        std::vector<std::pair<size_t, size_t>> range((size_t)prog->n_ops);
+        for (int64_t k = 0; k < prog->n_ops; k++) {
+            if (!is_gate_code(prog->ops[k].code))
+                throw Error(QFVM_ERR_VALUE, "qfvm_adjoint takes unitary ops only");
+            const size_t a = fwd.size();
+            lower_op(*s, prog, prog->ops[k], fwd);
+            range[(size_t)k] = {a, fwd.size()};
+        }
+        // one marker per differentiated op: the generator on the op's single target, its controls
+        // as a projector (grad_gate, gradiant.py:96-141)
+        std::vector<int> marker_of((size_t)prog->n_ops, -1);
+        int64_t begin = prog->n_ops;
+        std::vector<Gate> markers((size_t)n_grads);
+        for (int64_t g = 0; g < n_grads; g++) {
+            const int64_t k = grad_ops[g];
+            if (k < 0 || k >= prog->n_ops) throw Error(QFVM_ERR_INDEX, "gradient op index out of range");
+            if (marker_of[(size_t)k] >= 0) throw Error(QFVM_ERR_VALUE, "op differentiated twice");
+            const auto r = range[(size_t)k];
+            if (r.second - r.first != 1 || fwd[r.first].kind == G_MULTI)
+                throw Error(QFVM_ERR_VALUE, "only one-target rotations can be differentiated");
+            Gate m;
+            m.kind = G_GRAD;
+            m.target = fwd[r.first].target;
+            m.ctrls = fwd[r.first].ctrls;
+            m.cmask = fwd[r.first].cmask;
+            switch (grad_gen[g]) {
+                case 'X': m.gen = GEN_X; break;
+                case 'Y': m.gen = GEN_Y; break;
+                case 'Z': m.gen = GEN_Z; break;
+                default: throw Error(QFVM_ERR_VALUE, "gradient generators are X, Y or Z");
+            }
+            m.slot = (int)g;
+            markers[(size_t)g] = m;
+            marker_of[(size_t)k] = (int)g;
+            begin = std::min(begin, k);
+        }
+        apply_gates(*s, fwd);
+        void* lam = s->ensure_psi2();
+        const double e = H.groups.empty() ? 0.0 : apply_hamiltonian(*s, H, lam);
+        if (H.groups.empty()) QFVM_HIP(hipMemsetAsync(lam, 0, s->size * s->amp_bytes(), s->stream));
+        if (expectation) *expectation = e;
+        if (n_grads == 0) {
+            QFVM_HIP(hipStreamSynchronize(s->stream));
+            return;
+        }
+        double* gdev = reinterpret_cast<double*>(s->ensure_aux((size_t)n_grads * sizeof(double)));
+        QFVM_HIP(hipMemsetAsync(gdev, 0, (size_t)n_grads * sizeof(double), s->stream));
+        // reverse sweep (gradiant.py:160-172): the marker of op k sits before U_k^dagger, where both
+        // states are taken just after U_k
+        std::vector<Gate> rev;
+        for (int64_t k = prog->n_ops - 1; k >= begin; k--) {
+            if (marker_of[(size_t)k] >= 0) rev.push_back(markers[(size_t)marker_of[(size_t)k]]);
+            const auto r = range[(size_t)k];
+            for (size_t j = r.second; j > r.first; j--) rev.push_back(dagger(fwd[j - 1]));
+        }
+        apply_gates2(*s, rev, lam, gdev);
+        double* hbuf = s->ensure_pinned((size_t)n_grads * sizeof(double));
+        QFVM_HIP(hipMemcpyAsync(hbuf, gdev, (size_t)n_grads * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+        std::memcpy(grads, hbuf, (size_t)n_grads * sizeof(double));
     });
 }
 

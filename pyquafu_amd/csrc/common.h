@@ -78,8 +78,12 @@ enum GateKind : int {
     G_REAL = 1,   // real 2x2 (H, RY)
     G_DIAG = 2,   // diag(d0, d1)
     G_SWAP = 3,   // X
-    G_MULTI = 4   // dense 2^k x 2^k on k targets (row bit j <-> targets[j])
+    G_MULTI = 4,  // dense 2^k x 2^k on k targets (row bit j <-> targets[j])
+    G_GRAD = 5    // gradient marker of the two-state reverse sweep (changes no amplitude)
 };
+
+// generator of a gradient marker: the Pauli P of a rotation exp(-i theta P / 2) on `target`
+enum GradGen : int { GEN_X = 1, GEN_Y = 2, GEN_Z = 3 };
 
 struct Gate {
     int kind = G_DENSE;
@@ -89,6 +93,8 @@ struct Gate {
     cd m[4] = {1.0, 0.0, 0.0, 1.0};  // DENSE / REAL: m00 m01 m10 m11; DIAG: m[0], m[1]
     std::vector<int> targets;      // G_MULTI
     std::vector<cd> mk;            // G_MULTI row-major 4^k
+    int gen = 0;                   // G_GRAD: GradGen
+    int slot = -1;                 // G_GRAD: output index; the marker yields Im <lam| P_ctrl |psi>
 };
 
 // per-kernel statistics (algorithmic bytes = what the kernel must read + write)
@@ -113,6 +119,9 @@ struct State {
     size_t host_pinned_bytes = 0;
     void* ops_buf = nullptr;  // fused-pass op arrays
     size_t ops_bytes = 0;
+    void* aux_buf = nullptr;  // Hamiltonian terms and gradient outputs
+    size_t aux_bytes = 0;
+    void* psi2 = nullptr;     // second state of the reverse sweep (allocated on first use)
     // execution options (qfvm_set_option); tile_bits / low_bits of 0 pick the defaults
     int fusion = 1;
     int tile_bits = 0;
@@ -131,6 +140,8 @@ struct State {
     void* ensure_scratch(size_t bytes);
     double* ensure_pinned(size_t bytes);
     void* ensure_ops(size_t bytes);
+    void* ensure_aux(size_t bytes);
+    void* ensure_psi2();
     int stat_index(const char* name);
     void resolve_events();  // fold finished event pairs into kstats (call after a stream sync)
 };
@@ -143,6 +154,29 @@ struct KernelScope {
     hipEvent_t a = nullptr, b = nullptr;
     KernelScope(State& st, const char* name, double bytes);
     ~KernelScope();
+};
+
+// Points the state's kernels at another buffer of the same shape for the scope's lifetime.
+struct BufferScope {
+    State& s;
+    void* saved;
+    BufferScope(State& st, void* buf) : s(st), saved(st.psi) { s.psi = buf; }
+    ~BufferScope() { s.psi = saved; }
+};
+
+// Pauli-sum observable in kernel form. (P psi)[i] = i^{#Y} (-1)^popc(j & (y|z)) psi[j], j = i ^ flip,
+// so every term is (flip, zy, c * i^{#Y}); terms with equal flip masks share one group and one read.
+struct HamTerm {
+    uint64_t zy;
+    double cre, cim;
+};
+struct HamGroup {
+    uint64_t flip;
+    int32_t first, count;
+};
+struct Hamiltonian {
+    std::vector<HamGroup> groups;
+    std::vector<HamTerm> terms;
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -158,6 +192,13 @@ void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double
 void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale);
 void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b);
 void sample_tree(State& s, const double* host_u, int64_t shots, uint64_t* host_out);
+// Re <psi|H|psi>; when `out` is not null, H psi is written there (a buffer shaped like the state)
+double apply_hamiltonian(State& s, const Hamiltonian& H, void* out);
+// grads_dev[g.slot] = Im <lam| P (controls projected) |psi> for one G_GRAD marker, over the whole state
+void launch_grad_marker(State& s, const Gate& g, const void* lam, double* grads_dev);
+// sums column c of a [cols x tiles] partial table into grads_dev[slot_of[c]] (fixed order)
+void launch_column_sums(State& s, const double* partials, uint64_t tiles, int cols, const int* slot_of,
+                        double* grads_dev);
 
 inline unsigned grid_for(uint64_t work, unsigned block, uint64_t cap = (1ull << 20)) {
     uint64_t g = (work + block - 1) / block;

@@ -21,7 +21,8 @@ enum FKind : int32_t {
     F_DIAG = 2,   // diag(d0, d1) on group bit g
     F_SWAP = 3,   // X on group bit g
     F_PHASE = 4,  // diag(d0, d1) on a qubit outside the group: g = that qubit's global position
-    F_FLUSH = 5   // apply the pending X relabeling of the group bits in g (a 4-bit mask) to the data
+    F_FLUSH = 5,  // apply the pending X relabeling of the group bits in g (a 4-bit mask) to the data
+    F_GRAD = 6    // gradient marker (stage groups only): g = GradGen, see FOpHost
 };
 
 // X gates without a control among the group bits move no data: each lane keeps a 4-bit relabeling
@@ -40,12 +41,17 @@ struct FOpHost {
     int32_t g;
     uint32_t cm_group;  // control bits among the 4 group bits
     uint32_t flags;     // FFlags
-    uint64_t cm_ext;    // control qubits outside the group (global positions)
+    uint64_t cm_ext;    // control qubits outside the group (global positions); F_GRAD: all controls
     cd m[4];            // DENSE/REAL: m00 m01 m10 m11; DIAG/PHASE: d0 = m[0], d1 = m[3]
+    int32_t target;     // F_GRAD: the generator's qubit (global position)
+    int32_t slot;       // F_GRAD: output index
 };
 
+// A group is either a compute group (4 tile qubits and their gates) or, in the two-state reverse
+// sweep, a stage: gradient markers evaluated on the tiles of both states between two compute groups.
 struct FGroup {
-    int S[FUSE_GROUP_BITS];  // global qubits of the group (ascending)
+    int S[FUSE_GROUP_BITS];  // global qubits of the group (ascending); unused by a stage
+    bool stage = false;
     std::vector<FOpHost> ops;
 };
 
@@ -68,7 +74,9 @@ struct PlanItem {
 };
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg);
-// Runs a plan: fused passes on the LDS-tiled kernel, single gates on the per-gate kernels.
-void execute_plan(State& s, const std::vector<PlanItem>& plan);
+// Runs a plan: fused passes on the LDS-tiled kernel, single gates on the per-gate kernels. With
+// `lam` set, every gate acts on both s.psi and lam (the reverse sweep) and the plan's gradient
+// markers write grads_dev[slot].
+void execute_plan(State& s, const std::vector<PlanItem>& plan, void* lam = nullptr, double* grads_dev = nullptr);
 
 }  // namespace qfvm

@@ -32,6 +32,7 @@ struct PassDesc {
     int n, b, L;
     int q[FUSE_MAX_TILE_BITS];  // global qubit of local bit k (ascending; q[k] = k for k < L)
     int ngroups;
+    uint32_t stage_mask;        // bit gi: group gi is a gradient stage (two-state passes)
     int gbits[FUSE_MAX_GROUPS][FUSE_GROUP_BITS];  // local bits of each group, ascending
     int gop[FUSE_MAX_GROUPS + 1];                  // op ranges in the op array
 };
@@ -230,9 +231,75 @@ __device__ __forceinline__ void apply_op(V2<R> (&v)[16], int& px, const FOpDev<R
     }
 }
 
+__device__ __forceinline__ double wave_total(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Gradient stage of a two-state pass: for each marker (kind F_GRAD; g = generator, cm_group = the
+// generator's local tile bit for X / Y or its global qubit for Z, cm_ext = control mask, flags =
+// column) the tile's share of Im(conj(lam) P psi) over control-satisfied amplitudes goes to
+// partials[column * tiles + tile]. Markers are taken 8 at a time; each thread walks the elements it
+// loads (l = lh + tid), then the wavefronts' sums are added in a fixed order.
 template <typename R>
-__global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, const PassDesc pd,
-                                                       const FOpDev<R>* __restrict__ ops) {
+__device__ void grad_stage(const V2<R>* phi, const V2<R>* lam, int T, uint64_t g_tid, const PassDesc& pd,
+                           const FOpDev<R>* __restrict__ ops, int cnt, double* red, double* __restrict__ partials,
+                           uint64_t tiles) {
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    for (int c0 = 0; c0 < cnt; c0 += 8) {
+        const int cn = min(8, cnt - c0);
+        double acc[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] = 0.0;
+        for (int lh = 0; lh < T; lh += NT) {
+            const int e = lh + tid;
+            if (e >= T) break;
+            const uint64_t gi = g_tid | local_to_global(lh, pd);
+            const V2<R> p = phi[swz(e)], l = lam[swz(e)];
+            const double w = (double)l.x * p.y - (double)l.y * p.x;  // Im(conj(l) p)
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (k < cn) {
+                    const FOpDev<R>& o = ops[c0 + k];
+                    double v;
+                    if (o.g == GEN_Z) {
+                        v = ((gi >> o.cm_group) & 1ull) ? -w : w;
+                    } else {
+                        const int lb = (int)o.cm_group;
+                        const V2<R> q = phi[swz(e ^ (1 << lb))];
+                        if (o.g == GEN_X) {
+                            v = (double)l.x * q.y - (double)l.y * q.x;
+                        } else {  // Y: (Y psi)[e] = +i q where bit lb of e is set, -i q where clear
+                            const double re = (double)l.x * q.x + (double)l.y * q.y;
+                            v = ((e >> lb) & 1) ? re : -re;
+                        }
+                    }
+                    acc[k] += ((gi & o.cm_ext) == o.cm_ext) ? v : 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double t = wave_total(acc[k]);
+            if (k < cn && lane == 0) red[k * 8 + wave] = t;
+        }
+        __syncthreads();
+        if (tid < cn) {
+            double sum = 0.0;
+            for (int w = 0; w < (NT >> 6); w++) sum += red[tid * 8 + w];
+            partials[(uint64_t)ops[c0 + tid].flags * tiles + blockIdx.x] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// NS = 1: one state. NS = 2: the reverse sweep; every gate acts on psi and psi1 alike (their tiles
+// sit side by side in LDS) and stage groups evaluate gradient markers between compute groups.
+template <typename R, int NS>
+__global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<R>* __restrict__ psi1,
+                                                       const PassDesc pd, const FOpDev<R>* __restrict__ ops,
+                                                       double* __restrict__ partials, uint64_t tiles) {
     extern __shared__ unsigned char lds_raw[];
     V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
     const int b = pd.b;
@@ -247,55 +314,77 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, con
     const int tid = threadIdx.x;
     const int NT = blockDim.x;  // 64 .. 512, a power of two
     const uint64_t g_tid = base | local_to_global(tid, pd);
-    if (sizeof(R) == 8 && T >= NT && NT >= 128) {
-        // direct HBM -> LDS loads (gfx950 global_load_lds_dwordx4): LDS slot s = lh + tid receives
-        // element swz(s) = lh + swz(tid) (the swizzle only mixes bits 0-6, and lh is a multiple of
-        // NT >= 128), so no VGPRs hold the tile in flight
-        const uint64_t g_sw = base | local_to_global(swz(tid), pd);
-        const int wave = tid >> 6;
+#pragma unroll
+    for (int st = 0; st < NS; st++) {
+        const V2<R>* src_state = st ? psi1 : psi;
+        V2<R>* tl = tile + st * T;
+        if (sizeof(R) == 8 && T >= NT && NT >= 128) {
+            // direct HBM -> LDS loads (gfx950 global_load_lds_dwordx4): LDS slot s = lh + tid
+            // receives element swz(s) = lh + swz(tid) (the swizzle only mixes bits 0-6, and lh is a
+            // multiple of NT >= 128), so no VGPRs hold the tile in flight
+            const uint64_t g_sw = base | local_to_global(swz(tid), pd);
+            const int wave = tid >> 6;
 #pragma unroll 16
-        for (int lh = 0; lh < T; lh += NT) {
-            const void* src = (const void*)(psi + (g_sw | local_to_global(lh, pd)));
-            auto* dst = (__attribute__((address_space(3))) void*)(tile + lh + wave * 64);
-            if constexpr (sizeof(R) == 8) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+            for (int lh = 0; lh < T; lh += NT) {
+                const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
+                auto* dst = (__attribute__((address_space(3))) void*)(tl + lh + wave * 64);
+                if constexpr (sizeof(R) == 8) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+            }
+        } else {
+#pragma unroll 16
+            for (int lh = 0; lh < T; lh += NT)
+                if (lh + tid < T) tl[swz(lh + tid)] = src_state[g_tid | local_to_global(lh, pd)];
         }
-        __builtin_amdgcn_s_waitcnt(0);
-    } else {
-#pragma unroll 16
-        for (int lh = 0; lh < T; lh += NT)
-            if (lh + tid < T) tile[swz(lh + tid)] = psi[g_tid | local_to_global(lh, pd)];
     }
+    if (sizeof(R) == 8 && T >= NT && NT >= 128) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     const int rows = T >> 4;
     for (int gi = 0; gi < pd.ngroups; gi++) {
-        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];
         const int o_begin = pd.gop[gi], o_end = pd.gop[gi + 1];
+        if constexpr (NS == 2) {
+            if ((pd.stage_mask >> gi) & 1u) {
+                double* red = reinterpret_cast<double*>(tile + 2 * T);
+                grad_stage<R>(tile, tile + T, T, g_tid, pd, ops + o_begin, o_end - o_begin, red, partials, tiles);
+                continue;  // grad_stage ends with a barrier
+            }
+        }
+        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];
         for (int r = tid; r < rows; r += NT) {
             const int l0 = ins_bit(ins_bit(ins_bit(ins_bit(r, s0), s1), s2), s3);
             int a[16];
             row_slots(a, l0, s0, s1, s2, s3, std::make_integer_sequence<int, 16>{});
-            V2<R> v[16];
-            gather_seq<R>(v, tile, a, std::make_integer_sequence<int, 16>{});
             const uint64_t G0 = base | local_to_global(l0, pd);
-            int px = 0;
-            for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, px, ops[oi], G0);
-            const int xo = swz(((px & 1) << s0) | (((px >> 1) & 1) << s1) | (((px >> 2) & 1) << s2) |
-                               (((px >> 3) & 1) << s3));
-            scatter_seq<R>(v, tile, a, xo, std::make_integer_sequence<int, 16>{});
+            for (int st = 0; st < NS; st++) {
+                V2<R>* tl = tile + st * T;
+                V2<R> v[16];
+                gather_seq<R>(v, tl, a, std::make_integer_sequence<int, 16>{});
+                int px = 0;
+                for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, px, ops[oi], G0);
+                const int xo = swz(((px & 1) << s0) | (((px >> 1) & 1) << s1) | (((px >> 2) & 1) << s2) |
+                                   (((px >> 3) & 1) << s3));
+                scatter_seq<R>(v, tl, a, xo, std::make_integer_sequence<int, 16>{});
+            }
         }
         __syncthreads();
     }
 
+#pragma unroll
+    for (int st = 0; st < NS; st++) {
+        V2<R>* dst_state = st ? psi1 : psi;
+        const V2<R>* tl = tile + st * T;
 #pragma unroll 4
-    for (int lh = 0; lh < T; lh += NT) {
-        if (lh + tid < T) psi[g_tid | local_to_global(lh, pd)] = tile[swz(lh + tid)];
+        for (int lh = 0; lh < T; lh += NT) {
+            if (lh + tid < T) dst_state[g_tid | local_to_global(lh, pd)] = tl[swz(lh + tid)];
+        }
     }
 }
 
 // ------------------------------------------------------------------------------------------------
+// Appends the pass's ops to h and, for two-state passes, the output slot of each marker column to
+// slots (column c of the pass = its c-th marker).
 template <typename R>
-static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R>>& h) {
+static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R>>& h, std::vector<int>& slots) {
     const int b = (int)p.Q.size();
     int L = 0;
     while (L < b && p.Q[L] == L) L++;
@@ -313,6 +402,27 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
     const int first = (int)h.size();
     for (int gi = 0; gi < pd.ngroups; gi++) {
         const FGroup& G = p.groups[gi];
+        if (G.stage) {
+            pd.stage_mask |= 1u << gi;
+            pd.gop[gi] = (int)h.size() - first;
+            for (int j = 0; j < FUSE_GROUP_BITS; j++) pd.gbits[gi][j] = 0;
+            for (const FOpHost& o : G.ops) {
+                FOpDev<R> d{};
+                d.kind = F_GRAD;
+                d.g = o.g;
+                if (o.g == GEN_Z) {
+                    d.cm_group = (uint32_t)o.target;
+                } else {
+                    if (local_of[o.target] < 0) throw Error(QFVM_ERR_RUNTIME, "gradient marker outside the tile");
+                    d.cm_group = (uint32_t)local_of[o.target];
+                }
+                d.cm_ext = o.cm_ext;
+                d.flags = (uint32_t)slots.size();
+                slots.push_back(o.slot);
+                h.push_back(d);
+            }
+            continue;
+        }
         for (int j = 0; j < FUSE_GROUP_BITS; j++) {
             if (local_of[G.S[j]] < 0) throw Error(QFVM_ERR_RUNTIME, "group qubit outside the tile");
             pd.gbits[gi][j] = local_of[G.S[j]];
@@ -336,58 +446,94 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
     return pd;
 }
 
-template <typename R>
-static void execute_typed(State& s, const std::vector<PlanItem>& plan) {
-    // encode every pass's ops into one array, upload once, then launch the items in order
+template <typename R, int NS>
+static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam, double* grads_dev) {
+    // encode every pass's ops (and marker slots) into one array, upload once, then launch the
+    // items in order
     std::vector<FOpDev<R>> h;
     std::vector<PassDesc> descs;
     std::vector<size_t> first;
+    std::vector<int> slots_all;
+    std::vector<size_t> slot_first, slot_count;
     for (const PlanItem& it : plan) {
         if (!it.fused) continue;
         first.push_back(h.size());
-        descs.push_back(encode_pass<R>(s, it.pass, h));
+        std::vector<int> slots;
+        descs.push_back(encode_pass<R>(s, it.pass, h, slots));
+        if (NS == 1 && !slots.empty()) throw Error(QFVM_ERR_RUNTIME, "gradient marker in a one-state pass");
+        slot_first.push_back(slots_all.size());
+        slot_count.push_back(slots.size());
+        slots_all.insert(slots_all.end(), slots.begin(), slots.end());
     }
     FOpDev<R>* dops = nullptr;
+    int* dslots = nullptr;
     if (!h.empty()) {
-        const size_t bytes = h.size() * sizeof(FOpDev<R>);
+        const size_t obytes = h.size() * sizeof(FOpDev<R>);
+        const size_t soff = (obytes + 255) / 256 * 256;
+        const size_t bytes = soff + slots_all.size() * sizeof(int);
         QFVM_HIP(hipStreamSynchronize(s.stream));  // the pinned stage may still feed an earlier copy
-        dops = reinterpret_cast<FOpDev<R>*>(s.ensure_ops(bytes));
-        void* stage = s.ensure_pinned(bytes);
-        std::memcpy(stage, h.data(), bytes);
-        QFVM_HIP(hipMemcpyAsync(dops, stage, bytes, hipMemcpyHostToDevice, s.stream));
+        unsigned char* dev = reinterpret_cast<unsigned char*>(s.ensure_ops(bytes));
+        unsigned char* stage = reinterpret_cast<unsigned char*>(s.ensure_pinned(bytes));
+        std::memcpy(stage, h.data(), obytes);
+        if (!slots_all.empty()) std::memcpy(stage + soff, slots_all.data(), slots_all.size() * sizeof(int));
+        QFVM_HIP(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, s.stream));
+        dops = reinterpret_cast<FOpDev<R>*>(dev);
+        dslots = reinterpret_cast<int*>(dev + soff);
     }
+    V2<R>* psi = reinterpret_cast<V2<R>*>(s.psi);
+    V2<R>* psi1 = reinterpret_cast<V2<R>*>(lam);
     size_t k = 0;
     for (const PlanItem& it : plan) {
         if (!it.fused) {
             launch_gate(s, it.gate);
+            if (NS == 2) {
+                BufferScope other(s, lam);
+                launch_gate(s, it.gate);
+            }
             continue;
         }
         const PassDesc& pd = descs[k];
         const FOpDev<R>* ops = dops + first[k];
+        const int cols = (int)slot_count[k];
+        const int* slot_of = dslots + slot_first[k];
         k++;
-        const size_t lds = ((size_t)1 << pd.b) * sizeof(V2<R>);
+        const size_t lds = NS * ((size_t)1 << pd.b) * sizeof(V2<R>) + (NS == 2 ? 64 * sizeof(double) : 0);
+        if (lds > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "fused pass tiles exceed 160 KiB of LDS");
         if (lds > 65536) {
             static bool attr_set = false;
             if (!attr_set) {
-                QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<R>),
+                QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<R, NS>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
                 attr_set = true;
             }
         }
         const uint64_t tiles = 1ull << (s.n - pd.b);
-        KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
-        hipLaunchKernelGGL(k_fused_pass<R>, dim3((unsigned)tiles), dim3(threads), lds, s.stream,
-                           reinterpret_cast<V2<R>*>(s.psi), pd, ops);
-        QFVM_LAUNCHED();
+        // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
+        // between passes stages its matrix there)
+        double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;
+        {
+            KernelScope ks(s, NS == 1 ? "fused_pass" : "fused_pass2", 2.0 * NS * (double)s.size * sizeof(V2<R>));
+            hipLaunchKernelGGL((k_fused_pass<R, NS>), dim3((unsigned)tiles), dim3(threads), lds, s.stream, psi, psi1,
+                               pd, ops, partials, tiles);
+            QFVM_LAUNCHED();
+        }
+        if (cols > 0) launch_column_sums(s, partials, tiles, cols, slot_of, grads_dev);
     }
 }
 
-void execute_plan(State& s, const std::vector<PlanItem>& plan) {
-    if (s.dtype == QFVM_C128)
-        execute_typed<double>(s, plan);
-    else
-        execute_typed<float>(s, plan);
+void execute_plan(State& s, const std::vector<PlanItem>& plan, void* lam, double* grads_dev) {
+    if (lam) {
+        if (s.dtype == QFVM_C128)
+            execute_typed<double, 2>(s, plan, lam, grads_dev);
+        else
+            execute_typed<float, 2>(s, plan, lam, grads_dev);
+    } else {
+        if (s.dtype == QFVM_C128)
+            execute_typed<double, 1>(s, plan, nullptr, nullptr);
+        else
+            execute_typed<float, 1>(s, plan, nullptr, nullptr);
+    }
 }
 
 }  // namespace qfvm

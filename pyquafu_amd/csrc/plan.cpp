@@ -24,13 +24,21 @@ inline int popc(uint64_t x) { return __builtin_popcountll(x); }
 
 struct GInfo {
     uint64_t N = 0, D = 0;
-    bool diag = false, multi = false;
+    bool diag = false, multi = false, grad = false;
     cd d0, d1;
 };
 
 GInfo info_of(const Gate& g) {
     GInfo r;
     r.D = g.cmask;
+    if (g.kind == G_GRAD) {  // the generator's roles: Z is diagonal, X and Y need the target in a tile
+        r.grad = true;
+        if (g.gen == GEN_Z)
+            r.D |= 1ull << g.target;
+        else
+            r.N = 1ull << g.target;
+        return r;
+    }
     if (g.kind == G_MULTI) {
         r.multi = true;
         for (int t : g.targets) r.N |= 1ull << t;
@@ -50,8 +58,13 @@ GInfo info_of(const Gate& g) {
 
 struct GroupB {
     uint64_t S = 0, N = 0, A = 0;
+    bool stage = false;
     std::vector<int> gates;
 };
+
+// two gates (or a gate and a group) commute when no qubit is non-diagonal for one and touched by
+// the other
+inline bool commutes(const GInfo& g, const GroupB& G) { return !((g.N & G.A) || ((g.N | g.D) & G.N)); }
 
 FOpHost convert(const Gate& g, const GInfo& gi, const int S[4]) {
     FOpHost o{};
@@ -87,7 +100,9 @@ FOpHost convert(const Gate& g, const GInfo& gi, const int S[4]) {
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg) {
     const int b = std::min(cfg.tile_bits, n);
-    const int L = std::min(cfg.low_bits, b);
+    // a tile keeps at least one free slot when it does not hold every qubit, so the first waiting
+    // gate always fits and every pass makes progress
+    const int L = b < n ? std::min(cfg.low_bits, b - 1) : b;
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     std::vector<GInfo> info(gates.size());
     for (size_t i = 0; i < gates.size(); i++) info[i] = info_of(gates[i]);
@@ -114,17 +129,22 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             if (take && popc(newQ) > b) take = false;
             int place = -1;
             if (take) {
+                // latest group that can hold the gate (a stage holds markers, a compute group gates),
+                // moving back only past groups it commutes with
                 for (int j = (int)groups.size() - 1; j >= 0; j--) {
                     const GroupB& G = groups[j];
-                    if ((G.S & g.N) == g.N || popc(G.S | g.N) <= FUSE_GROUP_BITS) {
+                    const bool fits = g.grad ? G.stage
+                                             : (!G.stage && ((G.S & g.N) == g.N || popc(G.S | g.N) <= FUSE_GROUP_BITS));
+                    if (fits) {
                         place = j;
                         break;
                     }
-                    if ((g.N & G.A) || (g.D & G.N)) break;  // cannot move before group j
+                    if (!commutes(g, G)) break;  // cannot move before group j
                 }
                 if (place < 0) {
                     if ((int)groups.size() < cfg.max_groups) {
                         groups.emplace_back();
+                        groups.back().stage = g.grad;
                         place = (int)groups.size() - 1;
                     } else {
                         take = false;
@@ -138,7 +158,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                 continue;
             }
             GroupB& G = groups[place];
-            G.S |= g.N;
+            if (!G.stage) G.S |= g.N;
             G.N |= g.N;
             G.A |= A;
             G.gates.push_back((int)i);
@@ -151,6 +171,22 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
         for (int q = 0; q < n; q++)
             if ((Q >> q) & 1) it.pass.Q.push_back(q);
         for (GroupB& G : groups) {
+            if (G.stage) {
+                FGroup fg;
+                fg.stage = true;
+                for (int j = 0; j < FUSE_GROUP_BITS; j++) fg.S[j] = -1;
+                for (int gi : G.gates) {
+                    FOpHost o{};
+                    o.kind = F_GRAD;
+                    o.g = gates[gi].gen;
+                    o.cm_ext = gates[gi].cmask;
+                    o.target = gates[gi].target;
+                    o.slot = gates[gi].slot;
+                    fg.ops.push_back(o);
+                }
+                it.pass.groups.push_back(std::move(fg));
+                continue;
+            }
             for (int q : it.pass.Q) {
                 if (popc(G.S) >= FUSE_GROUP_BITS) break;
                 G.S |= 1ull << q;
@@ -185,18 +221,41 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
     return plan;
 }
 
+static FuseConfig config_of(const State& s, int states) {
+    FuseConfig cfg;
+    // the two-state sweep holds both tiles in LDS, so its default tile is half as large
+    const int def = (s.dtype == QFVM_C128 ? 12 : 13) - (states - 1);
+    cfg.tile_bits = s.tile_bits > 0 ? s.tile_bits - (states - 1) : def;
+    cfg.tile_bits = std::max(FUSE_GROUP_BITS, std::min(cfg.tile_bits, FUSE_MAX_TILE_BITS));
+    cfg.low_bits = s.low_bits > 0 ? s.low_bits : 6;
+    cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
+    return cfg;
+}
+
 void apply_gates(State& s, const std::vector<Gate>& gates) {
     if (gates.empty()) return;
     if (!s.fusion || s.n < 6) {
         for (const Gate& g : gates) launch_gate(s, g);
         return;
     }
-    FuseConfig cfg;
-    cfg.tile_bits = s.tile_bits > 0 ? s.tile_bits : (s.dtype == QFVM_C128 ? 12 : 13);
-    cfg.tile_bits = std::max(FUSE_GROUP_BITS, std::min(cfg.tile_bits, FUSE_MAX_TILE_BITS));
-    cfg.low_bits = s.low_bits > 0 ? s.low_bits : 6;
-    cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
-    execute_plan(s, plan_gates(gates, s.n, cfg));
+    execute_plan(s, plan_gates(gates, s.n, config_of(s, 1)));
+}
+
+void apply_gates2(State& s, const std::vector<Gate>& gates, void* lam, double* grads_dev) {
+    if (gates.empty()) return;
+    if (!s.fusion || s.n < 6) {
+        for (const Gate& g : gates) {
+            if (g.kind == G_GRAD) {
+                launch_grad_marker(s, g, lam, grads_dev);
+                continue;
+            }
+            launch_gate(s, g);
+            BufferScope other(s, lam);
+            launch_gate(s, g);
+        }
+        return;
+    }
+    execute_plan(s, plan_gates(gates, s.n, config_of(s, 2)), lam, grads_dev);
 }
 
 // ------------------------------------------------------------------------------------------------

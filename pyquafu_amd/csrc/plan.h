@@ -9,5 +9,8 @@ namespace qfvm {
 
 // Applies `gates` in order to the state (asynchronously on s.stream).
 void apply_gates(State& s, const std::vector<Gate>& gates);
+// The reverse sweep: applies `gates` in order to both s.psi and `lam`; a G_GRAD marker writes
+// grads_dev[slot] = Im <lam| P (controls projected) |psi> for the two states at its position.
+void apply_gates2(State& s, const std::vector<Gate>& gates, void* lam, double* grads_dev);
 
 }  // namespace qfvm

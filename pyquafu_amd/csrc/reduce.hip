@@ -1,5 +1,6 @@
 // Reductions over the state: squared norm, probabilities, Pauli-string expectations, marginal
-// probabilities of measured qubits; and the projective collapse / reset kernels.
+// probabilities of measured qubits, Hamiltonian expectations and gradient markers; and the
+// projective collapse / reset kernels.
 //
 // Sums are deterministic: a fixed grid (independent of the device), a fixed per-thread stride
 // order, a fixed wavefront shuffle tree and a fixed block tree, then one block that adds the
@@ -7,6 +8,7 @@
 // reduction.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "common.h"
 
@@ -256,6 +258,140 @@ void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_stat
     else
         hipLaunchKernelGGL(k_collapse<float>, dim3(grid), dim3(256), 0, s.stream, reinterpret_cast<float2*>(s.psi),
                            s.size, ql, meas_state, (float)scale);
+    QFVM_LAUNCHED();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Observables and gradients.
+//
+// k_hamiltonian: per amplitude i, (H psi)[i] = sum over flip groups of psi[i ^ flip] times the sum
+// of the group's signed coefficients; adds Re(conj(psi[i]) (H psi)[i]) to the expectation and, when
+// `out` is given, stores (H psi)[i]. Bytes: one write plus one read per flip group (neighbouring
+// flips are served from cache).
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_hamiltonian(const V2<R>* __restrict__ psi, V2<R>* __restrict__ out,
+                                                            uint64_t size, const HamGroup* __restrict__ groups,
+                                                            int ngroups, const HamTerm* __restrict__ terms,
+                                                            double* partial) {
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        double sr = 0.0, si = 0.0;
+        for (int g = 0; g < ngroups; g++) {
+            const uint64_t j = i ^ groups[g].flip;
+            double cr = 0.0, ci = 0.0;
+            const int t1 = groups[g].first + groups[g].count;
+            for (int t = groups[g].first; t < t1; t++) {
+                const bool neg = __popcll(j & terms[t].zy) & 1;
+                cr += neg ? -terms[t].cre : terms[t].cre;
+                ci += neg ? -terms[t].cim : terms[t].cim;
+            }
+            const V2<R> p = psi[j];
+            sr += cr * (double)p.x - ci * (double)p.y;
+            si += cr * (double)p.y + ci * (double)p.x;
+        }
+        if (out) out[i] = mk<R>((R)sr, (R)si);
+        const V2<R> a = psi[i];
+        acc += (double)a.x * sr + (double)a.y * si;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// One gradient marker over the whole state: sum over control-satisfied i of
+// Im(conj(lam[i]) (P psi)[i]); X: (P psi)[i] = psi[i ^ t]; Y: -i psi[i | t] where bit t of i is clear,
+// +i psi[i & ~t] where set; Z: (-1)^bit psi[i].
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_grad_marker(const V2<R>* __restrict__ psi, const V2<R>* __restrict__ lam,
+                                                            uint64_t size, int gen, int q, uint64_t cmask,
+                                                            double* partial) {
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tb = 1ull << q;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        if ((i & cmask) != cmask) continue;
+        const V2<R> l = lam[i];
+        const bool bit = (i & tb) != 0;
+        double v;
+        if (gen == GEN_Z) {
+            const V2<R> p = psi[i];
+            const double w = (double)l.x * p.y - (double)l.y * p.x;
+            v = bit ? -w : w;
+        } else {
+            const V2<R> p = psi[i ^ tb];
+            if (gen == GEN_X) {
+                v = (double)l.x * p.y - (double)l.y * p.x;
+            } else {
+                const double re = (double)l.x * p.x + (double)l.y * p.y;
+                v = bit ? re : -re;
+            }
+        }
+        acc += v;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// Column c of a [cols x tiles] table of per-tile partials -> grads[slot_of[c]], in a fixed order.
+__global__ void __launch_bounds__(RED_BLOCK) k_column_sums(const double* __restrict__ partials, uint64_t tiles,
+                                                            const int* __restrict__ slot_of, double* grads) {
+    const double* col = partials + (uint64_t)blockIdx.x * tiles;
+    double acc = 0.0;
+    for (uint64_t k = threadIdx.x; k < tiles; k += blockDim.x) acc += col[k];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) grads[slot_of[blockIdx.x]] = acc;
+}
+
+double apply_hamiltonian(State& s, const Hamiltonian& H, void* out) {
+    const size_t gbytes = H.groups.size() * sizeof(HamGroup), tbytes = H.terms.size() * sizeof(HamTerm);
+    const size_t toff = (gbytes + 255) / 256 * 256;
+    unsigned char* dev = reinterpret_cast<unsigned char*>(s.ensure_aux(toff + tbytes + 256));
+    if (!H.groups.empty()) {
+        QFVM_HIP(hipStreamSynchronize(s.stream));  // the pinned stage may still feed an earlier copy
+        unsigned char* stage = reinterpret_cast<unsigned char*>(s.ensure_pinned(toff + tbytes));
+        std::memcpy(stage, H.groups.data(), gbytes);
+        std::memcpy(stage + toff, H.terms.data(), tbytes);
+        QFVM_HIP(hipMemcpyAsync(dev, stage, toff + tbytes, hipMemcpyHostToDevice, s.stream));
+    }
+    double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
+    const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(s.size, RED_BLOCK));
+    const auto* g = reinterpret_cast<const HamGroup*>(dev);
+    const auto* t = reinterpret_cast<const HamTerm*>(dev + toff);
+    KernelScope ks(s, "hamiltonian", (double)s.size * s.amp_bytes() * (1.0 + (out ? 1.0 : 0.0) + H.groups.size()));
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_hamiltonian<double>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const double2*>(s.psi), reinterpret_cast<double2*>(out), s.size, g,
+                           (int)H.groups.size(), t, partial);
+    else
+        hipLaunchKernelGGL(k_hamiltonian<float>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const float2*>(s.psi), reinterpret_cast<float2*>(out), s.size, g,
+                           (int)H.groups.size(), t, partial);
+    QFVM_LAUNCHED();
+    return finish_sum(s, partial, grid);
+}
+
+void launch_grad_marker(State& s, const Gate& g, const void* lam, double* grads_dev) {
+    double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
+    const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(s.size, RED_BLOCK));
+    KernelScope ks(s, "grad_marker", 2.0 * (double)s.size * s.amp_bytes());
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_grad_marker<double>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const double2*>(s.psi), reinterpret_cast<const double2*>(lam), s.size,
+                           g.gen, g.target, g.cmask, partial);
+    else
+        hipLaunchKernelGGL(k_grad_marker<float>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const float2*>(s.psi), reinterpret_cast<const float2*>(lam), s.size,
+                           g.gen, g.target, g.cmask, partial);
+    QFVM_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s.stream, partial, (int)grid, grads_dev + g.slot);
+    QFVM_LAUNCHED();
+}
+
+void launch_column_sums(State& s, const double* partials, uint64_t tiles, int cols, const int* slot_of,
+                        double* grads_dev) {
+    if (cols <= 0) return;
+    hipLaunchKernelGGL(k_column_sums, dim3((unsigned)cols), dim3(RED_BLOCK), 0, s.stream, partials, tiles, slot_of,
+                       grads_dev);
     QFVM_LAUNCHED();
 }
 

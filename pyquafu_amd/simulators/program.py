@@ -195,7 +195,16 @@ def compile_circuit(qc) -> Program:
 
 def compile_ops(ops, n_qubits: int) -> Program:
     """Compile a plain list of gates (applyop_statevec and the gradient code use this)."""
+    return compile_gates(ops, n_qubits)[0]
+
+
+def compile_gates(gates, n_qubits: int):
+    """Compile a list of gates; also returns, per gate, the index of the op row it became (-1 for
+    barriers, delays and identities)."""
     b = ProgramBuilder()
-    for op in ops:
-        _emit_pyop(b, op)
-    return b.program(n_qubits=n_qubits)
+    rows = []
+    for op in gates:
+        r = len(b.rows)
+        kind = _emit_pyop(b, op)
+        rows.append(r if kind == "gate" else -1)
+    return b.program(n_qubits=n_qubits), rows

@@ -85,3 +85,71 @@ def random_classical_program(n: int, rng, n_blocks: int = 4):
     p = b.program(n_qubits=n, n_cbits=n)
     p.n_random = n_random
     return p
+
+
+def _rot(gen: str, t: float) -> np.ndarray:
+    c, s = np.cos(t / 2), np.sin(t / 2)
+    if gen == "X":
+        return np.array([[c, -1j * s], [-1j * s, c]])
+    if gen == "Y":
+        return np.array([[c, -s], [s, c]], dtype=complex)
+    return np.diag([np.exp(-0.5j * t), np.exp(0.5j * t)])
+
+
+def rotation_program(n: int, n_ops: int, rng, max_ctrl: int = 2):
+    """A circuit of fixed gates and differentiable rotations (named RX / RY / RZ, and rotations with
+    controls as GENERAL matrices). Returns (make, thetas, grad_ops, gens): make(thetas) rebuilds the
+    program, grad_ops[k] is the op row of rotation k, gens[k] its generator letter."""
+    plan = []
+    for _ in range(n_ops):
+        kind = int(rng.integers(0, 6))
+        if kind <= 1:
+            plan.append(("named", str(rng.choice(["X", "Y", "Z"])), [int(rng.integers(n))], 0))
+        elif kind == 2 and n >= 2 and max_ctrl > 0:
+            nc = int(rng.integers(1, min(max_ctrl, n - 1) + 1))
+            q = [int(x) for x in rng.choice(n, nc + 1, replace=False)]
+            plan.append(("ctrl", str(rng.choice(["X", "Y", "Z"])), q, nc))
+        elif kind == 3 and n >= 2:
+            q = [int(x) for x in rng.choice(n, 2, replace=False)]
+            plan.append(("fixed", int(rng.choice([CX, CZ])), q, 1))
+        elif kind == 4 and n >= 2:
+            q = [int(x) for x in rng.choice(n, 2, replace=False)]
+            plan.append(("multi", random_unitary(4, rng), q, 0))
+        else:
+            plan.append(("u", random_unitary(2, rng), [int(rng.integers(n))], 0))
+    n_theta = sum(1 for p in plan if p[0] in ("named", "ctrl"))
+    thetas = rng.uniform(-np.pi, np.pi, n_theta)
+    code = {"X": RX, "Y": RY, "Z": RZ}
+
+    def make(th):
+        b = ProgramBuilder()
+        k = 0
+        for kind, a, q, nc in plan:
+            if kind == "named":
+                b.add(code[a], q, param=float(th[k]))
+                k += 1
+            elif kind == "ctrl":
+                b.add(GENERAL, q, n_ctrl=nc, matrix=_rot(a, float(th[k])))
+                k += 1
+            elif kind == "fixed":
+                b.add(a, q, n_ctrl=nc)
+            elif kind == "multi":
+                b.add(MULTI, q, matrix=a)
+            else:
+                b.add(GENERAL, q, matrix=a)
+        return b.program(n_qubits=n)
+
+    grad_ops = [i for i, p in enumerate(plan) if p[0] in ("named", "ctrl")]
+    gens = "".join(p[1] for p in plan if p[0] in ("named", "ctrl"))
+    return make, thetas, grad_ops, gens
+
+
+def random_pauli_terms(n: int, n_terms: int, rng):
+    """Random Pauli strings (no repeated qubit) with real coefficients."""
+    terms, coeffs = [], []
+    for _ in range(n_terms):
+        k = int(rng.integers(1, min(3, n) + 1))
+        pos = [int(x) for x in rng.choice(n, k, replace=False)]
+        terms.append(("".join(str(rng.choice(["X", "Y", "Z"])) for _ in pos), pos))
+        coeffs.append(float(rng.normal()))
+    return terms, coeffs

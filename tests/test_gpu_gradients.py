@@ -1,0 +1,187 @@
+"""Adjoint gradients and Hamiltonian expectations on the GPU, through the C-ABI, against the oracle
+(oracle/gradients.py) and against parameter shift / finite differences on the engine itself."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import gradients as og
+from pyquafu_amd import _native, workloads
+from tests.progs import random_pauli_terms, rotation_program
+
+pytestmark = pytest.mark.gpu
+
+
+def engine_adjoint(prog, n, ops, gens, terms, coeffs, dtype="c128", options=()):
+    dev = _native.DeviceState(n, dtype)
+    try:
+        for k, v in options:
+            dev.set_option(k, v)
+        dev.init_zero()
+        return dev.adjoint(prog, ops, gens, terms, coeffs)
+    finally:
+        dev.close()
+
+
+@pytest.mark.parametrize("n,fusion", [(3, 1), (5, 0), (6, 1), (8, 0), (9, 1), (12, 1)])
+def test_adjoint_matches_oracle(n, fusion):
+    rng = np.random.Generator(np.random.PCG64(1200 + n))
+    make, thetas, ops, gens = rotation_program(n, 60, rng)
+    terms, coeffs = random_pauli_terms(n, 6, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, options=[("fusion", fusion)])
+    assert abs(e - e_want) < 1e-11
+    assert np.abs(got - want).max() < 1e-10
+
+
+@pytest.mark.parametrize("tile,low,groups", [(6, 2, 2), (8, 3, 4), (10, 6, 24), (13, 6, 12)])
+def test_adjoint_tile_shapes(tile, low, groups):
+    # small tiles and few groups force many passes, stages next to every kind of group and X / Y
+    # markers on qubits far above the low bits
+    n = 13
+    rng = np.random.Generator(np.random.PCG64(1300 + tile))
+    make, thetas, ops, gens = rotation_program(n, 80, rng)
+    terms, coeffs = random_pauli_terms(n, 5, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs,
+                            options=[("tile_bits", tile), ("low_bits", low), ("max_groups", groups)])
+    assert abs(e - e_want) < 1e-11
+    assert np.abs(got - want).max() < 1e-10
+
+
+def test_adjoint_complex64():
+    n = 10
+    rng = np.random.Generator(np.random.PCG64(1400))
+    make, thetas, ops, gens = rotation_program(n, 50, rng)
+    terms, coeffs = random_pauli_terms(n, 4, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, dtype="c64")
+    assert abs(e - e_want) < 1e-4
+    assert np.abs(got - want).max() < 1e-4
+
+
+def test_expect_hamiltonian_matches_oracle():
+    n = 11
+    rng = np.random.Generator(np.random.PCG64(1500))
+    terms, coeffs = random_pauli_terms(n, 20, rng)
+    coeffs = [c * (1 + 0.5j) for c in coeffs]  # the real part of a complex-weighted sum
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    dev = _native.DeviceState(n)
+    dev.load(psi)
+    got = dev.expect_hamiltonian(terms, coeffs)
+    dev.close()
+    assert abs(got - og.expectation(psi, n, terms, coeffs)) < 1e-12
+
+
+def test_adjoint_equals_engine_parameter_shift_on_ansatz():
+    # hardware-efficient ansatz (RY, RZ layers + CZ ring) with the TFIM Hamiltonian, 12 qubits
+    from pyquafu_amd.algorithms.gradients.gradients import adjoint_program, hamiltonian_terms
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    n, layers = 12, 2
+    qc, values = workloads.hea_ansatz(n, layers, parametrised=True)
+    ham = workloads.tfim_hamiltonian(n)
+    terms, coeffs = hamiltonian_terms(ham)
+    prog, ops, gens, _ = adjoint_program(qc)
+    dev = _native.DeviceState(n)
+    dev.init_zero()
+    grads, e = dev.adjoint(prog, ops, gens, terms, coeffs)
+    ps = np.zeros(len(ops))
+    for k, row in enumerate(ops):
+        vals = []
+        for shift in (np.pi / 2, -np.pi / 2):
+            p2 = compile_circuit(qc)
+            p2.ops["param"][row] += shift
+            dev.init_zero()
+            dev.apply(p2)
+            vals.append(dev.expect_hamiltonian(terms, coeffs))
+        ps[k] = (vals[0] - vals[1]) / 2
+    dev.close()
+    assert np.abs(grads - ps).max() < 1e-10
+
+
+def _reference_gradient_circuit():
+    # the circuit of the reference's test_gradient (tests/quafu/algorithms/varational_test.py:68-89)
+    from pyquafu_amd.circuits import QuantumCircuit
+    from pyquafu_amd.elements import CXGate, HGate, Parameter, RYGate
+
+    pq = QuantumCircuit(4)
+    theta = [Parameter("theta_%d" % i, i + 1) for i in range(4)]
+    for i in range(4):
+        pq << HGate(i)
+    for i in range(3):
+        pq << CXGate(i, i + 1)
+    for i in range(4):
+        pq << RYGate(i, theta[i])
+    for i in range(3):
+        pq << CXGate(i, i + 1)
+    for i in range(4):
+        pq << RYGate(i, theta[i])
+    pq << RYGate(2, theta[0] * theta[1] - 3 * theta[0])
+    pq.get_parameter_grads()
+    return pq
+
+
+def test_reference_gradient_scenario():
+    from pyquafu_amd.algorithms import Hamiltonian, PauliOp
+    from pyquafu_amd.algorithms.gradients import grad_adjoint, grad_finit_diff, grad_para_shift
+
+    pq = _reference_gradient_circuit()
+    hamil = Hamiltonian([PauliOp("X1 X2"), PauliOp("Z0 Z3")])
+    for values in (None, [0.1, 0.73, 2.1, 3.2]):
+        if values is not None:
+            pq._update_params(values)
+        fd = grad_finit_diff(pq, hamil, h=1e-6)
+        ps = grad_para_shift(pq, hamil)
+        ad = grad_adjoint(pq, hamil)
+        assert len(ad) == 4
+        for i in range(4):
+            assert math.isclose(ad[i], ps[i], rel_tol=1e-9, abs_tol=1e-12)
+            assert abs(fd[i] - ps[i]) < 1e-5
+
+
+def test_reference_controlled_adjoint_scenario():
+    # the circuit of the reference's test_ctrl_adjoint (varational_test.py:107-127)
+    from pyquafu_amd.algorithms import Hamiltonian, PauliOp
+    from pyquafu_amd.algorithms.gradients import grad_adjoint, grad_finit_diff
+    from pyquafu_amd.circuits import QuantumCircuit
+    from pyquafu_amd.elements import CRYGate, CXGate, HGate, Parameter
+
+    pq = QuantumCircuit(4)
+    theta = [Parameter("theta_%d" % i, i + 1) for i in range(4)]
+    for i in range(4):
+        pq << HGate(i)
+    for i in range(3):
+        pq << CXGate(i, i + 1)
+        pq << CRYGate(i, i + 1, theta[i])
+    for i in range(3):
+        pq << CXGate(i, i + 1)
+    for i in range(3):
+        pq << CRYGate(i + 1, i, theta[i])
+    pq.get_parameter_grads()
+    hamil = Hamiltonian([PauliOp("X1 X2"), PauliOp("Z0 Z3")])
+    fd = grad_finit_diff(pq, hamil, h=1e-6)
+    ad = grad_adjoint(pq, hamil)
+    assert len(ad) == len(fd) == 3
+    for i in range(3):
+        assert abs(fd[i] - ad[i]) < 1e-5
+
+
+def test_adjoint_errors():
+    n = 3
+    rng = np.random.Generator(np.random.PCG64(1600))
+    make, thetas, ops, gens = rotation_program(n, 10, rng)
+    prog = make(thetas)
+    dev = _native.DeviceState(n)
+    with pytest.raises(_native.EngineError):
+        dev.adjoint(prog, [len(prog.ops)], "X", [("Z", [0])], [1.0])  # op index out of range
+    with pytest.raises(_native.EngineError):
+        dev.adjoint(prog, ops[:1], "Q", [("Z", [0])], [1.0])  # unknown generator
+    with pytest.raises(_native.EngineError):
+        dev.adjoint(prog, ops[:1], gens[:1], [("ZZ", [0, 0])], [1.0])  # repeated qubit in a term
+    dev.close()
