@@ -107,6 +107,81 @@ def copy_calibration() -> int:
     return 0
 
 
+def adjoint_leg(n: int, layers: int, fd_params: int = 8):
+    """Adjoint gradient vs parameter shift on the same engine: 24-qubit complex128 hardware-efficient
+    ansatz (per layer RY and RZ on every qubit, CZ ring; parameters U[-pi, pi], PCG64 seed 20250003),
+    loss <H> of the periodic TFIM H = sum Z_i Z_i+1 + 0.7 sum X_i. One full gradient = the loss and
+    all 2 * n * layers partials. Parameter shift runs all 2P shifted circuits (each: reset, the
+    fused forward passes, one Hamiltonian pass); central differences (h = 1e-6) on the first
+    `fd_params` parameters are a sanity bound."""
+    from pyquafu_amd import _native, workloads
+    from pyquafu_amd.algorithms.gradients.gradients import adjoint_program, hamiltonian_terms
+
+    qc, values = workloads.hea_ansatz(n, layers, parametrised=True)
+    ham = workloads.tfim_hamiltonian(n)
+    terms, coeffs = hamiltonian_terms(ham)
+    prog, ops, gens, _ = adjoint_program(qc)
+    dev = _native.DeviceState(n, "c128")
+
+    def adjoint_once():
+        dev.init_zero()
+        return dev.adjoint(prog, ops, gens, terms, coeffs)
+
+    adjoint_once()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        grads, e = adjoint_once()
+        times.append(time.perf_counter() - t0)
+    t_adj = float(np.median(times))
+    dev.set_profiling(True)
+    adjoint_once()
+    prof = dev.profile(reset=True)
+    dev.set_profiling(False)
+
+    def energy(p):
+        dev.init_zero()
+        dev.apply(p)
+        return dev.expect_hamiltonian(terms, coeffs)
+
+    energy(prog)
+    t0 = time.perf_counter()
+    ps = np.zeros(len(ops))
+    for k, row in enumerate(ops):
+        theta = float(prog.ops["param"][row])
+        prog.ops["param"][row] = theta + np.pi / 2
+        plus = energy(prog)
+        prog.ops["param"][row] = theta - np.pi / 2
+        minus = energy(prog)
+        prog.ops["param"][row] = theta
+        ps[k] = (plus - minus) / 2
+    t_ps = time.perf_counter() - t0
+    fd = np.zeros(fd_params)
+    for k in range(fd_params):
+        row = ops[k]
+        theta = float(prog.ops["param"][row])
+        prog.ops["param"][row] = theta + 1e-6
+        plus = energy(prog)
+        prog.ops["param"][row] = theta - 1e-6
+        minus = energy(prog)
+        prog.ops["param"][row] = theta
+        fd[k] = (plus - minus) / 2e-6
+    dev.close()
+    return {
+        "workload": f"{n}q c128 hardware-efficient ansatz, {layers} layers (RY+RZ on every qubit, CZ ring), "
+                    f"TFIM h=0.7 periodic, {len(ops)} parameters",
+        "adjoint_grads_per_s": round(1.0 / t_adj, 3),
+        "adjoint_ms": round(t_adj * 1e3, 3),
+        "param_shift_grads_per_s": round(1.0 / t_ps, 4),
+        "param_shift_ms": round(t_ps * 1e3, 1),
+        "speedup_vs_param_shift": round(t_ps / t_adj, 1),
+        "max_abs_diff_vs_param_shift": float(np.abs(grads - ps).max()),
+        "max_abs_diff_vs_central_fd": float(np.abs(grads[:fd_params] - fd).max()),
+        "expectation": e,
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in prof.items()},
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +198,9 @@ def main(argv=None) -> int:
     ap.add_argument("--max-groups", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the streaming calibration kernel for the HBM counters")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-parameter-shift leg")
+    ap.add_argument("--adjoint-n", type=int, default=24)
+    ap.add_argument("--adjoint-layers", type=int, default=8)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the statevector "
@@ -213,6 +291,11 @@ def main(argv=None) -> int:
         "norm_error": norm_err,
     }
     dev.close()
+    if not args.no_adjoint:
+        print(f"[bench] adjoint leg: {args.adjoint_n}q, {args.adjoint_layers} layers", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_leg(args.adjoint_n, args.adjoint_layers)
+    else:
+        out["adjoint"] = None
     if not args.no_cpu:
         per_layer = n + n // 2 + (n - 1) // 2 + args.crz
         print(f"[bench] CPU baseline: {per_layer} gates on the oracle", file=sys.stderr, flush=True)
