@@ -213,6 +213,7 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *   "tile_bits"   qubits per fused tile b (4..14; 0 = default: 12 for complex128, 13 for complex64)
  *   "low_bits"    low qubits always in a tile L (0 = default 6)
  *   "max_groups"  4-qubit register groups per fused pass (1..24, default 12)
+ *   "pass_cost"   compute budget of one fused pass in quarter dense-gate units (0 = default)
  * Results do not depend on the options beyond floating-point rounding.
  */
 int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
@@ -221,9 +222,11 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
  * The execution plan qfvm_apply would run for `prog` on an n-qubit state, as JSON (host only, no
  * device needed; no counterpart in the reference). Fused passes list their tile qubits and, per
  * 4-qubit group, the ops in execution order: {"kind", "g", "cmg", "cme", "m": [re, im] x 4}; other
- * items carry the gate. Lets the planner's reordering be checked without a GPU.
+ * items carry the gate. Lets the planner's reordering be checked without a GPU. Arguments of 0
+ * pick the defaults, as for qfvm_set_option.
  */
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
+                   int32_t pass_cost,
                    const qfvm_program* prog, char* buf, int64_t cap, int64_t* len);
 
 #ifdef __cplusplus

@@ -681,6 +681,9 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
         } else if (k == "max_groups") {
             if (value < 1 || value > 24) throw Error(QFVM_ERR_VALUE, "max_groups must be in 1..24");
             s->max_groups = (int)value;
+        } else if (k == "pass_cost") {
+            if (value < 0 || value > (1 << 20)) throw Error(QFVM_ERR_VALUE, "pass_cost must be in 0..2^20");
+            s->pass_cost = (int)value;
         } else {
             throw Error(QFVM_ERR_VALUE, "unknown option '" + k + "'");
         }
@@ -688,7 +691,7 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
 }
 
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
-                   const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
+                   int32_t pass_cost, const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
     return guarded([&] {
         if (n < 0 || n > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
         check_program(prog);
@@ -705,6 +708,7 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
         cfg.tile_bits = tile_bits > 0 ? tile_bits : (tmp.dtype == QFVM_C128 ? 12 : 13);
         cfg.low_bits = low_bits > 0 ? low_bits : 6;
         cfg.max_groups = max_groups > 0 ? max_groups : 12;
+        cfg.max_cost = pass_cost > 0 ? pass_cost : 0;
         std::vector<PlanItem> plan = plan_gates(gates, n, cfg);
         std::string j = "{\"items\": [";
         char t[512];

@@ -127,6 +127,7 @@ struct State {
     int tile_bits = 0;
     int low_bits = 0;
     int max_groups = 12;
+    int pass_cost = 0;  // fused-pass compute budget (quarter dense-gate units; 0 = unlimited)
     // counters: always on; event timing only when `profiling`
     bool profiling = false;
     std::vector<std::pair<std::string, KStat>> kstats;

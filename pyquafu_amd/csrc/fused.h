@@ -19,17 +19,23 @@ enum FKind : int32_t {
     F_DENSE = 0,  // complex 2x2 on group bit g
     F_REAL = 1,   // real 2x2 on group bit g
     F_DIAG = 2,   // diag(d0, d1) on group bit g
-    F_SWAP = 3,   // X on group bit g
+    F_SWAP = 3,   // X on group bit g with a control among the group bits
     F_PHASE = 4,  // diag(d0, d1) on a qubit outside the group: g = that qubit's global position
-    F_FLUSH = 5,  // apply the pending X relabeling of the group bits in g (a 4-bit mask) to the data
-    F_GRAD = 6    // gradient marker (stage groups only): g = GradGen, see FOpHost
+    F_FLUSH = 5,  // apply the pending relabeling of group bit g to the data (mask in pxmask)
+    F_GRAD = 6,   // gradient marker (stage groups only): g = GradGen, see FOpHost
+    F_RELABEL = 7 // host only: an X folded into the relabeling of bit g (listed, never uploaded)
 };
 
-// X gates without a control among the group bits move no data: each lane keeps a 4-bit relabeling
-// px (register member P holds logical member P ^ px) that is applied when the row is written back
-// to LDS. A later gate whose target bit is pending uses swapped coefficients (flag FL_TARGET_PX); a
-// gate with a control on a pending bit is preceded by F_FLUSH, which moves the data.
-enum FFlags : uint32_t { FL_TARGET_PX = 1 };
+// X gates without a control among the group bits move no data. An X on group bit j whose controls
+// outside the group are one qubit c (or none) flips bit j of the row's relabeling exactly where bit
+// c of the row's index is set, so the relabeling of bit j after a run of such X gates is the parity
+// of (row index & M_j), plus a constant for uncontrolled X. The host folds those X gates into the
+// masks M_j (F_RELABEL, not uploaded); register member P then holds logical member P ^ px with px_j
+// = parity(G0 & M_j) ^ c_j, applied when the row is written back to LDS (the group's final masks
+// are in the pass descriptor). A later gate whose target bit is pending uses swapped coefficients
+// where px_j is set (flag FL_TARGET_PX, mask in pxmask, constant in FL_PX_CONST); a gate with a
+// control on a pending bit is preceded by F_FLUSH, which moves the data.
+enum FFlags : uint32_t { FL_TARGET_PX = 1, FL_PX_CONST = 2 };
 
 constexpr int FUSE_GROUP_BITS = 4;
 constexpr int FUSE_MAX_GROUPS = 24;
@@ -45,6 +51,7 @@ struct FOpHost {
     cd m[4];            // DENSE/REAL: m00 m01 m10 m11; DIAG/PHASE: d0 = m[0], d1 = m[3]
     int32_t target;     // F_GRAD: the generator's qubit (global position)
     int32_t slot;       // F_GRAD: output index
+    uint64_t pxmask;    // FL_TARGET_PX / F_FLUSH: relabel parity mask of the op's bit g
 };
 
 // A group is either a compute group (4 tile qubits and their gates) or, in the two-state reverse
@@ -53,6 +60,8 @@ struct FGroup {
     int S[FUSE_GROUP_BITS];  // global qubits of the group (ascending); unused by a stage
     bool stage = false;
     std::vector<FOpHost> ops;
+    uint64_t pxm[FUSE_GROUP_BITS] = {0, 0, 0, 0};  // relabel masks at the end of the group
+    uint32_t pxc = 0;                              // relabel constants (bit j for group bit j)
 };
 
 struct FPass {
@@ -64,6 +73,7 @@ struct FuseConfig {
     int tile_bits;   // b
     int low_bits;    // L
     int max_groups;
+    int max_cost;    // compute budget of a pass in quarter dense-gate units (0: unlimited)
 };
 
 // One item of an execution plan: a fused pass or a single gate for the per-gate kernels.

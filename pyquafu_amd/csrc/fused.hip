@@ -25,6 +25,7 @@ struct FOpDev {
     uint32_t cm_group;
     uint32_t flags;
     uint64_t cm_ext;
+    uint64_t pxmask;  // relabel parity mask of bit g (FL_TARGET_PX, F_FLUSH)
     R m[8];
 };
 
@@ -35,7 +36,14 @@ struct PassDesc {
     uint32_t stage_mask;        // bit gi: group gi is a gradient stage (two-state passes)
     int gbits[FUSE_MAX_GROUPS][FUSE_GROUP_BITS];  // local bits of each group, ascending
     int gop[FUSE_MAX_GROUPS + 1];                  // op ranges in the op array
+    uint64_t gpxm[FUSE_MAX_GROUPS][FUSE_GROUP_BITS];  // relabel masks at each group's end
+    uint32_t gpxc[FUSE_MAX_GROUPS];                   // relabel constants at each group's end
 };
+
+// bit j of the row's relabeling: parity(G0 & mask) ^ constant
+__device__ __forceinline__ int px_bit(uint64_t G0, uint64_t mask, uint32_t c) {
+    return (int)((__popcll(G0 & mask) ^ c) & 1u);
+}
 
 __device__ __forceinline__ int swz(int l) { return l ^ ((l >> 4) & 7); }
 
@@ -92,13 +100,13 @@ __device__ __forceinline__ void pairs_seq(V2<R> (&v)[16], int cmg, V2<R> m00, V2
 
 // 2x2 kinds (dense, real, diagonal) and controlled X with group controls and outside controls.
 template <typename R, int KIND, int G>
-__device__ __forceinline__ void op_2x2(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, int px) {
+__device__ __forceinline__ void op_2x2(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, uint64_t G0) {
     V2<R> m00 = mk<R>(o.m[0], o.m[1]), m01 = mk<R>(o.m[2], o.m[3]);
     V2<R> m10 = mk<R>(o.m[4], o.m[5]), m11 = mk<R>(o.m[6], o.m[7]);
     if (o.flags & FL_TARGET_PX) {
-        // register pair (Q0, Q1) holds logical (L1, L0) where bit G of px is set: the matrix acts
-        // as [[m11, m10], [m01, m00]] on the registers
-        const bool sw = (px >> G) & 1;
+        // register pair (Q0, Q1) holds logical (L1, L0) where bit G of the row's relabeling is
+        // set: the matrix acts as [[m11, m10], [m01, m00]] on the registers
+        const bool sw = px_bit(G0, o.pxmask, (o.flags >> 1) & 1u);
         const V2<R> t00 = m00, t01 = m01, t10 = m10, t11 = m11;
         m00 = csel<R>(sw, t11, t00);
         m01 = csel<R>(sw, t10, t01);
@@ -115,8 +123,8 @@ __device__ __forceinline__ void op_2x2(V2<R> (&v)[16], const FOpDev<R>& o, bool 
     pairs_seq<R, KIND, G>(v, (int)o.cm_group, m00, m01, m10, m11, std::make_integer_sequence<int, 16>{});
 }
 
-// Moves the data of pending relabel bit G: pairs swap where bit G of px is set (bitwise blend, so
-// no select of registers is formed).
+// Moves the data of pending relabel bit G: pairs swap where the row's relabel bit G is set (bitwise
+// blend, so no select of registers is formed).
 template <typename R, int G, int M>
 __device__ __forceinline__ void flush_step(V2<R> (&v)[16], uint64_t msk) {
     if constexpr (!((M >> G) & 1)) {
@@ -144,10 +152,9 @@ __device__ __forceinline__ void flush_step(V2<R> (&v)[16], uint64_t msk) {
 }
 
 template <typename R, int G, int... Ms>
-__device__ __forceinline__ void flush_bit(V2<R> (&v)[16], int& px, std::integer_sequence<int, Ms...>) {
-    const uint64_t msk = 0ull - (uint64_t)((px >> G) & 1);
+__device__ __forceinline__ void flush_bit(V2<R> (&v)[16], int sw, std::integer_sequence<int, Ms...>) {
+    const uint64_t msk = 0ull - (uint64_t)sw;
     (flush_step<R, G, Ms>(v, msk), ...);
-    px &= ~(1 << G);
 }
 
 template <typename R, int M>
@@ -192,37 +199,38 @@ __device__ __forceinline__ uint64_t local_to_global(int l, const PassDesc& pd) {
 }
 
 template <typename R, int KIND>
-__device__ __forceinline__ void op_kind(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, int px) {
+__device__ __forceinline__ void op_kind(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, uint64_t G0) {
     switch (o.g) {
-        case 0: op_2x2<R, KIND, 0>(v, o, ok, px); break;
-        case 1: op_2x2<R, KIND, 1>(v, o, ok, px); break;
-        case 2: op_2x2<R, KIND, 2>(v, o, ok, px); break;
-        default: op_2x2<R, KIND, 3>(v, o, ok, px); break;
+        case 0: op_2x2<R, KIND, 0>(v, o, ok, G0); break;
+        case 1: op_2x2<R, KIND, 1>(v, o, ok, G0); break;
+        case 2: op_2x2<R, KIND, 2>(v, o, ok, G0); break;
+        default: op_2x2<R, KIND, 3>(v, o, ok, G0); break;
     }
 }
 
 template <typename R>
-__device__ __forceinline__ void apply_op(V2<R> (&v)[16], int& px, const FOpDev<R>& o, uint64_t G0) {
+__device__ __forceinline__ void apply_op(V2<R> (&v)[16], const FOpDev<R>& o, uint64_t G0) {
     const bool ok = (G0 & o.cm_ext) == o.cm_ext;
     switch (o.kind) {
-        case F_DENSE: op_kind<R, F_DENSE>(v, o, ok, px); break;
-        case F_REAL: op_kind<R, F_REAL>(v, o, ok, px); break;
-        case F_DIAG: op_kind<R, F_DIAG>(v, o, ok, px); break;
-        case F_SWAP:
-            if (o.cm_group == 0) {
-                px ^= ok ? (1 << o.g) : 0;  // relabel only
-            } else if (o.cm_ext == 0) {
-                op_kind<R, F_SWAP>(v, o, ok, px);  // uniform register moves
-            } else {
-                op_kind<R, F_DENSE>(v, o, ok, px);  // X as a matrix, identity where outside controls fail
+        case F_DENSE: op_kind<R, F_DENSE>(v, o, ok, G0); break;
+        case F_REAL: op_kind<R, F_REAL>(v, o, ok, G0); break;
+        case F_DIAG: op_kind<R, F_DIAG>(v, o, ok, G0); break;
+        case F_SWAP:  // controls among the group bits (X without them is a relabeling, folded on the host)
+            if (o.cm_ext == 0)
+                op_kind<R, F_SWAP>(v, o, ok, G0);  // uniform register moves
+            else
+                op_kind<R, F_DENSE>(v, o, ok, G0);  // X as a matrix, identity where outside controls fail
+            break;
+        case F_FLUSH: {
+            const int sw = px_bit(G0, o.pxmask, (o.flags >> 1) & 1u);
+            switch (o.g) {
+                case 0: flush_bit<R, 0>(v, sw, std::make_integer_sequence<int, 16>{}); break;
+                case 1: flush_bit<R, 1>(v, sw, std::make_integer_sequence<int, 16>{}); break;
+                case 2: flush_bit<R, 2>(v, sw, std::make_integer_sequence<int, 16>{}); break;
+                default: flush_bit<R, 3>(v, sw, std::make_integer_sequence<int, 16>{}); break;
             }
             break;
-        case F_FLUSH:
-            if (o.g & 1) flush_bit<R, 0>(v, px, std::make_integer_sequence<int, 16>{});
-            if (o.g & 2) flush_bit<R, 1>(v, px, std::make_integer_sequence<int, 16>{});
-            if (o.g & 4) flush_bit<R, 2>(v, px, std::make_integer_sequence<int, 16>{});
-            if (o.g & 8) flush_bit<R, 3>(v, px, std::make_integer_sequence<int, 16>{});
-            break;
+        }
         default: {  // F_PHASE: one factor for the whole row, chosen by an index bit outside the group
             V2<R> d = csel<R>((G0 >> o.g) & 1ull, mk<R>(o.m[6], o.m[7]), mk<R>(o.m[0], o.m[1]));
             d = csel<R>(ok, d, mk<R>(1, 0));
@@ -359,10 +367,11 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
                 V2<R>* tl = tile + st * T;
                 V2<R> v[16];
                 gather_seq<R>(v, tl, a, std::make_integer_sequence<int, 16>{});
-                int px = 0;
-                for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, px, ops[oi], G0);
-                const int xo = swz(((px & 1) << s0) | (((px >> 1) & 1) << s1) | (((px >> 2) & 1) << s2) |
-                                   (((px >> 3) & 1) << s3));
+                for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, ops[oi], G0);
+                const uint32_t pc = pd.gpxc[gi];
+                const int xo = swz((px_bit(G0, pd.gpxm[gi][0], pc) << s0) | (px_bit(G0, pd.gpxm[gi][1], pc >> 1) << s1) |
+                                   (px_bit(G0, pd.gpxm[gi][2], pc >> 2) << s2) |
+                                   (px_bit(G0, pd.gpxm[gi][3], pc >> 3) << s3));
                 scatter_seq<R>(v, tl, a, xo, std::make_integer_sequence<int, 16>{});
             }
         }
@@ -428,13 +437,17 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
             pd.gbits[gi][j] = local_of[G.S[j]];
         }
         pd.gop[gi] = (int)h.size() - first;
+        for (int j = 0; j < FUSE_GROUP_BITS; j++) pd.gpxm[gi][j] = G.pxm[j];
+        pd.gpxc[gi] = G.pxc;
         for (const FOpHost& o : G.ops) {
+            if (o.kind == F_RELABEL) continue;  // folded into the parity masks
             FOpDev<R> d{};
             d.kind = o.kind;
             d.g = o.g;
             d.cm_group = o.cm_group;
             d.flags = o.flags;
             d.cm_ext = o.cm_ext;
+            d.pxmask = o.pxmask;
             for (int k = 0; k < 4; k++) {
                 d.m[2 * k] = (R)o.m[k].real();
                 d.m[2 * k + 1] = (R)o.m[k].imag();

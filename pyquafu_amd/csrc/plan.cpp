@@ -62,6 +62,18 @@ struct GroupB {
     std::vector<int> gates;
 };
 
+// register work of one gate in a fused pass, in quarter dense-gate units (measured per-op costs:
+// dense 2x2 = 4; real and diagonal about 3; an X is mostly a free relabeling)
+int gate_cost(const Gate& g, const GInfo& gi) {
+    if (gi.grad) return 2;
+    if (gi.diag) return 3;
+    switch (g.kind) {
+        case G_SWAP: return 1;
+        case G_REAL: return 3;
+        default: return 4;
+    }
+}
+
 // two gates (or a gate and a group) commute when no qubit is non-diagonal for one and touched by
 // the other
 inline bool commutes(const GInfo& g, const GroupB& G) { return !((g.N & G.A) || ((g.N | g.D) & G.N)); }
@@ -120,6 +132,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
         uint64_t Q = L >= 64 ? ~0ull : ((1ull << L) - 1);
         uint64_t blockN = 0, blockA = 0;
         std::vector<GroupB> groups;
+        int cost = 0;
         for (size_t i = first; i < gates.size(); i++) {
             if (done[i]) continue;
             const GInfo& g = info[i];
@@ -150,6 +163,15 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                         take = false;
                     }
                 }
+            }
+            if (take && cfg.max_cost > 0) {
+                // estimated register work of the gate (a new group adds its gather / scatter)
+                const int c = gate_cost(gates[i], g) + (place == (int)groups.size() - 1 && groups.back().gates.empty() ? 4 : 0);
+                if (cost + c > cfg.max_cost && cost > 0) {
+                    if (place == (int)groups.size() - 1 && groups.back().gates.empty()) groups.pop_back();
+                    break;  // the pass is full
+                }
+                cost += c;
             }
             if (!take) {
                 blockN |= g.N;
@@ -195,25 +217,45 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             int j = 0;
             for (int q = 0; q < n; q++)
                 if ((G.S >> q) & 1) fg.S[j++] = q;
-            // X gates without group controls become relabelings (fused.h); track the pending bits
-            uint32_t pending = 0;
+            // X gates without group controls and with at most one outside control become parity
+            // relabelings (fused.h): M[j] / C track the pending relabeling of group bit j
+            uint64_t M[FUSE_GROUP_BITS] = {0, 0, 0, 0};
+            uint32_t C = 0;
+            auto pending = [&](int j) { return M[j] != 0 || ((C >> j) & 1); };
             for (int gi : G.gates) {
                 FOpHost o = convert(gates[gi], info[gi], fg.S);
-                const uint32_t need = o.cm_group & pending;
-                if (need) {
-                    FOpHost f{};
-                    f.kind = F_FLUSH;
-                    f.g = (int)need;
-                    fg.ops.push_back(f);
-                    pending &= ~need;
+                for (int j = 0; j < FUSE_GROUP_BITS; j++) {
+                    if (((o.cm_group >> j) & 1) && pending(j)) {
+                        FOpHost f{};
+                        f.kind = F_FLUSH;
+                        f.g = j;
+                        f.pxmask = M[j];
+                        f.flags = ((C >> j) & 1) ? FL_PX_CONST : 0;
+                        fg.ops.push_back(f);
+                        M[j] = 0;
+                        C &= ~(1u << j);
+                    }
                 }
-                if (o.kind == F_SWAP) {
-                    if (o.cm_group == 0) pending |= 1u << o.g;
-                } else if (o.kind != F_PHASE && ((pending >> o.g) & 1)) {
-                    o.flags |= FL_TARGET_PX;
+                if (o.kind == F_SWAP && o.cm_group == 0) {
+                    if (popc(o.cm_ext) <= 1) {
+                        M[o.g] ^= o.cm_ext;
+                        if (o.cm_ext == 0) C ^= 1u << o.g;
+                        o.kind = F_RELABEL;  // listed for inspection, never uploaded
+                        fg.ops.push_back(o);
+                        continue;
+                    }
+                    // several outside controls: a physical X where they all hold (X commutes with
+                    // a pending relabeling of the same bit)
+                    o.kind = F_DENSE;
+                }
+                if (o.kind != F_SWAP && o.kind != F_PHASE && pending(o.g)) {
+                    o.flags |= FL_TARGET_PX | (((C >> o.g) & 1) ? FL_PX_CONST : 0);
+                    o.pxmask = M[o.g];
                 }
                 fg.ops.push_back(o);
             }
+            for (int j = 0; j < FUSE_GROUP_BITS; j++) fg.pxm[j] = M[j];
+            fg.pxc = C;
             it.pass.groups.push_back(std::move(fg));
         }
         plan.push_back(std::move(it));
@@ -229,6 +271,7 @@ static FuseConfig config_of(const State& s, int states) {
     cfg.tile_bits = std::max(FUSE_GROUP_BITS, std::min(cfg.tile_bits, FUSE_MAX_TILE_BITS));
     cfg.low_bits = s.low_bits > 0 ? s.low_bits : 6;
     cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
+    cfg.max_cost = s.pass_cost;
     return cfg;
 }
 

@@ -13,7 +13,7 @@ from pyquafu_amd.simulators.program import compile_circuit
 from tests.conftest import rand_state
 from tests.progs import random_gate_program
 
-F_DENSE, F_REAL, F_DIAG, F_SWAP, F_PHASE, F_FLUSH = range(6)
+F_DENSE, F_REAL, F_DIAG, F_SWAP, F_PHASE, F_FLUSH, F_GRAD, F_RELABEL = range(8)
 G_MULTI = 4
 
 
@@ -41,7 +41,7 @@ def replay(plan, prog, n, psi):
             assert len(set(S)) == 4 and set(S) <= set(Q)
             for op in G["ops"]:
                 if op["kind"] == F_FLUSH:  # moves data between registers only; no logical effect
-                    assert 0 < op["g"] < 16
+                    assert 0 <= op["g"] < 4
                     continue
                 m = [complex(*z) for z in op["m"]]
                 cm = int(op["cme"])
@@ -63,7 +63,7 @@ def replay(plan, prog, n, psi):
                 i0 = idx[(bit == 0) & ok]
                 i1 = i0 | (1 << q)
                 a, b = psi[i0].copy(), psi[i1].copy()
-                if op["kind"] == F_SWAP:
+                if op["kind"] in (F_SWAP, F_RELABEL):  # a relabeling is an X folded into parity masks
                     psi[i0], psi[i1] = b, a
                 else:
                     if op["kind"] == F_REAL:
