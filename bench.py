@@ -182,6 +182,67 @@ def adjoint_leg(n: int, layers: int, fd_params: int = 8):
     }
 
 
+def qft_leg(n: int, shots: int, samples: int = 100000):
+    """Precision study: the swap-free QFT (H, then CP(2 pi / 2^k) from the later qubits, for each
+    target in textbook order) of a seeded basis state |x> (x ~ U{0 .. 2^n - 1}, PCG64 seed
+    20250005), run in complex64 and complex128 side by side. Errors: max-abs over `samples` seeded
+    random amplitudes against the closed form 2^(-n/2) exp(2 pi i x rev(y) / 2^n) (integer phase,
+    float64 exp), and the fidelity |<psi64|psi128>|^2 over the whole state. Throughput: pass-model
+    GB/s per dtype and the sampler's shots/s on the complex128 state."""
+    from pyquafu_amd import _native, workloads
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    rng = np.random.Generator(np.random.PCG64(20250005))
+    x = int(rng.integers(0, 1 << n))
+    ys = rng.integers(0, 1 << n, size=samples, dtype=np.int64)
+    prog = compile_circuit(workloads.qft(n, x))
+    want = workloads.qft_amplitudes(n, x, ys)
+    out = {"workload": f"{n}q QFT of |x>, x={x}, {prog.n_ops} gates", "dtypes": {}}
+    devs = {}
+    for dtype in ("c64", "c128"):
+        try:
+            dev = _native.DeviceState(n, dtype)
+        except _native.EngineError as e:
+            out["dtypes"][dtype] = {"skipped": str(e)}
+            continue
+        dev.init_zero()
+        dev.apply(prog)  # warm-up (plans, op upload)
+        dev.init_zero()
+        dev.synchronize()
+        dev.profile(reset=True)
+        t0 = time.perf_counter()
+        dev.apply(prog)
+        dev.synchronize()
+        dt = time.perf_counter() - t0
+        c = dev.profile(reset=True)
+        by = sum(v["bytes"] for v in c.values())
+        got = dev.gather(ys)
+        out["dtypes"][dtype] = {
+            "ms": round(dt * 1e3, 2),
+            "launches": int(sum(v["launches"] for v in c.values())),
+            "GBps": round(by / dt / 1e9, 1),
+            "frac_of_peak": round(by / dt / 1e9 / PEAK_GBPS, 4),
+            "max_abs_err_sampled": float(np.abs(got - want).max()),
+            "norm_error": abs(dev.norm2() - 1.0),
+        }
+        devs[dtype] = dev
+    if len(devs) == 2:
+        ov = devs["c64"].inner(devs["c128"])
+        out["fidelity_c64_vs_c128"] = float(abs(ov) ** 2)
+    if "c128" in devs and shots:
+        u = np.random.Generator(np.random.PCG64(20250006)).random(shots)
+        dev = devs["c128"]
+        dev.sample(u[:1000])
+        t0 = time.perf_counter()
+        idx = dev.sample(u)
+        dt = time.perf_counter() - t0
+        out["sampler"] = {"shots": shots, "ms": round(dt * 1e3, 2), "shots_per_s": round(shots / dt, 1),
+                          "distinct": int(np.unique(idx).size)}
+    for dev in devs.values():
+        dev.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +262,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-parameter-shift leg")
     ap.add_argument("--adjoint-n", type=int, default=24)
     ap.add_argument("--adjoint-layers", type=int, default=8)
+    ap.add_argument("--no-qft", action="store_true", help="skip the 33-qubit QFT precision leg")
+    ap.add_argument("--qft-n", type=int, default=33)
+    ap.add_argument("--qft-shots", type=int, default=1000000)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the statevector "
@@ -296,6 +360,11 @@ def main(argv=None) -> int:
         out["adjoint"] = adjoint_leg(args.adjoint_n, args.adjoint_layers)
     else:
         out["adjoint"] = None
+    if not args.no_qft:
+        print(f"[bench] QFT precision leg: {args.qft_n}q, c64 and c128", file=sys.stderr, flush=True)
+        out["precision"] = qft_leg(args.qft_n, args.qft_shots)
+    else:
+        out["precision"] = None
     if not args.no_cpu:
         per_layer = n + n // 2 + (n - 1) // 2 + args.crz
         print(f"[bench] CPU baseline: {per_layer} gates on the oracle", file=sys.stderr, flush=True)

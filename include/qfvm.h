@@ -176,6 +176,20 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
                  const int32_t* len, const double* coef, int64_t n_terms, double* grads,
                  double* expectation);
 
+/*
+ * Amplitudes at the given indices (out: n re, im pairs, complex128). No counterpart in the
+ * reference, where the caller holds the whole state in host memory; here it avoids copying a
+ * state of up to 256 GiB to read a few amplitudes.
+ */
+int qfvm_gather(qfvm_handle h, const uint64_t* indices, int64_t n, double* out);
+
+/*
+ * Overlap <a|b> of two states with the same qubit count (dtypes may differ), as in the overlap
+ * checks of the reference's circuit tests (tests/quafu/circuits/building_circuit_test.py:47-172,
+ * np.vdot of two simulate() states), without copying either state to the host.
+ */
+int qfvm_inner(qfvm_handle a, qfvm_handle b, double* re, double* im);
+
 /* |amp|^2 for every basis state (probabilities(), statevector.hpp:1023-1032). */
 int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n);
 

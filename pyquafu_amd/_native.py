@@ -64,6 +64,8 @@ SIGNATURES = {
     "qfvm_expect_hamiltonian": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "qfvm_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram), _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64,
                                     _vp, _vp]),
+    "qfvm_gather": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
+    "qfvm_inner": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
     "qfvm_probabilities": (ctypes.c_int, [_vp, _vp, _u64]),
     "qfvm_sample": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "qfvm_norm2": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl)]),
@@ -255,6 +257,20 @@ class DeviceState:
                                  grads.ctypes.data, ctypes.byref(e)))
         del keep
         return grads[: ops.size], e.value
+
+    def gather(self, indices) -> np.ndarray:
+        """Amplitudes at `indices` (complex128)."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        out = np.empty(idx.size, dtype=np.complex128)
+        if idx.size:
+            check(lib().qfvm_gather(self._h, idx.ctypes.data, idx.size, out.ctypes.data))
+        return out
+
+    def inner(self, other: "DeviceState") -> complex:
+        """<self|other> on the device."""
+        re, im = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(lib().qfvm_inner(self._h, other._h, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
 
     def probabilities(self) -> np.ndarray:
         out = np.empty(1 << self.n, dtype=np.float64)

@@ -651,6 +651,27 @@ int qfvm_sample(qfvm_handle h, const double* uniforms, int64_t shots, uint64_t* 
     });
 }
 
+int qfvm_gather(qfvm_handle h, const uint64_t* indices, int64_t n, double* out) {
+    return guarded([&] {
+        State* s = need(h);
+        if (n < 0 || (n > 0 && (!indices || !out))) throw Error(QFVM_ERR_VALUE, "bad gather arrays");
+        for (int64_t k = 0; k < n; k++)
+            if (indices[k] >= s->size) throw Error(QFVM_ERR_INDEX, "amplitude index out of range");
+        if (n > 0) gather_amplitudes(*s, indices, (uint64_t)n, out);
+    });
+}
+
+int qfvm_inner(qfvm_handle a, qfvm_handle b, double* re, double* im) {
+    return guarded([&] {
+        State* sa = need(a);
+        State* sb = need(b);
+        if (!re || !im) throw Error(QFVM_ERR_VALUE, "null output");
+        if (sa->n != sb->n) throw Error(QFVM_ERR_VALUE, "states differ in qubit count");
+        if (sa->device != sb->device) throw Error(QFVM_ERR_VALUE, "states live on different devices");
+        reduce_inner(*sa, *sb, re, im);
+    });
+}
+
 int qfvm_norm2(qfvm_handle h, double* out) {
     return guarded([&] { *out = reduce_norm2(*need(h)); });
 }

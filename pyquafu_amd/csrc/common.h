@@ -197,6 +197,9 @@ void sample_tree(State& s, const double* host_u, int64_t shots, uint64_t* host_o
 double apply_hamiltonian(State& s, const Hamiltonian& H, void* out);
 // grads_dev[g.slot] = Im <lam| P (controls projected) |psi> for one G_GRAD marker, over the whole state
 void launch_grad_marker(State& s, const Gate& g, const void* lam, double* grads_dev);
+// amplitudes at host-given indices (out: re, im pairs) and <a|b> of two states of one size
+void gather_amplitudes(State& s, const uint64_t* host_idx, uint64_t n, double* host_out);
+void reduce_inner(State& a, State& b, double* re, double* im);
 // sums column c of a [cols x tiles] partial table into grads_dev[slot_of[c]] (fixed order)
 void launch_column_sums(State& s, const double* partials, uint64_t tiles, int cols, const int* slot_of,
                         double* grads_dev);

@@ -395,6 +395,86 @@ void launch_column_sums(State& s, const double* partials, uint64_t tiles, int co
     QFVM_LAUNCHED();
 }
 
+// ------------------------------------------------------------------------------------------------
+// Amplitudes at given indices, and the overlap <a|b> of two states of one size (any dtypes).
+template <typename R>
+__global__ void k_gather(const V2<R>* __restrict__ psi, const uint64_t* __restrict__ idx, uint64_t n,
+                         double2* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const V2<R> a = psi[idx[k]];
+        out[k] = make_double2((double)a.x, (double)a.y);
+    }
+}
+
+template <typename RA, typename RB>
+__global__ void __launch_bounds__(RED_BLOCK) k_inner(const V2<RA>* __restrict__ a, const V2<RB>* __restrict__ b,
+                                                      uint64_t size, double* partial) {
+    double re = 0.0, im = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        const V2<RA> x = a[i];
+        const V2<RB> y = b[i];
+        re += (double)x.x * (double)y.x + (double)x.y * (double)y.y;  // conj(x) y
+        im += (double)x.x * (double)y.y - (double)x.y * (double)y.x;
+    }
+    re = block_sum(re);
+    im = block_sum(im);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = re;
+        partial[gridDim.x + blockIdx.x] = im;
+    }
+}
+
+void gather_amplitudes(State& s, const uint64_t* host_idx, uint64_t n, double* host_out) {
+    const uint64_t chunk = 1ull << 22;
+    for (uint64_t b = 0; b < n; b += chunk) {
+        const uint64_t m = std::min(chunk, n - b);
+        unsigned char* dev = reinterpret_cast<unsigned char*>(s.ensure_scratch(m * (sizeof(uint64_t) + 16)));
+        uint64_t* didx = reinterpret_cast<uint64_t*>(dev);
+        double2* dout = reinterpret_cast<double2*>(dev + m * sizeof(uint64_t));
+        QFVM_HIP(hipMemcpyAsync(didx, host_idx + b, m * sizeof(uint64_t), hipMemcpyHostToDevice, s.stream));
+        if (s.dtype == QFVM_C128)
+            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(m, 256)), dim3(256), 0, s.stream,
+                               reinterpret_cast<const double2*>(s.psi), didx, m, dout);
+        else
+            hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(m, 256)), dim3(256), 0, s.stream,
+                               reinterpret_cast<const float2*>(s.psi), didx, m, dout);
+        QFVM_LAUNCHED();
+        QFVM_HIP(hipMemcpyAsync(host_out + 2 * b, dout, m * 16, hipMemcpyDeviceToHost, s.stream));
+        QFVM_HIP(hipStreamSynchronize(s.stream));
+    }
+}
+
+void reduce_inner(State& a, State& b, double* re, double* im) {
+    double* partial = reinterpret_cast<double*>(a.ensure_scratch((2 * RED_GRID + 2) * sizeof(double)));
+    const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(a.size, RED_BLOCK));
+    // b's stream finishes its work first; the kernel runs on a's stream
+    QFVM_HIP(hipStreamSynchronize(b.stream));
+    if (a.dtype == QFVM_C128 && b.dtype == QFVM_C128)
+        hipLaunchKernelGGL((k_inner<double, double>), dim3(grid), dim3(RED_BLOCK), 0, a.stream,
+                           reinterpret_cast<const double2*>(a.psi), reinterpret_cast<const double2*>(b.psi), a.size, partial);
+    else if (a.dtype == QFVM_C128)
+        hipLaunchKernelGGL((k_inner<double, float>), dim3(grid), dim3(RED_BLOCK), 0, a.stream,
+                           reinterpret_cast<const double2*>(a.psi), reinterpret_cast<const float2*>(b.psi), a.size, partial);
+    else if (b.dtype == QFVM_C128)
+        hipLaunchKernelGGL((k_inner<float, double>), dim3(grid), dim3(RED_BLOCK), 0, a.stream,
+                           reinterpret_cast<const float2*>(a.psi), reinterpret_cast<const double2*>(b.psi), a.size, partial);
+    else
+        hipLaunchKernelGGL((k_inner<float, float>), dim3(grid), dim3(RED_BLOCK), 0, a.stream,
+                           reinterpret_cast<const float2*>(a.psi), reinterpret_cast<const float2*>(b.psi), a.size, partial);
+    QFVM_LAUNCHED();
+    double* out = partial + 2 * grid;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, a.stream, partial, (int)grid, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, a.stream, partial + grid, (int)grid, out + 1);
+    QFVM_LAUNCHED();
+    double* h = a.ensure_pinned(2 * sizeof(double));
+    QFVM_HIP(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, a.stream));
+    QFVM_HIP(hipStreamSynchronize(a.stream));
+    *re = h[0];
+    *im = h[1];
+}
+
 void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b) {
     uint64_t qmask = 0, fmask = 0, mmask = 0;
     for (size_t k = 0; k < qbits.size(); k++) {

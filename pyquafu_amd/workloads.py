@@ -84,9 +84,10 @@ def qft(n: int, x: int, measure: bool = False):
     return qc
 
 
-def qft_amplitudes(n: int, x: int) -> np.ndarray:
-    """Closed form of qft(n, x)'s output (float64 / complex128), exact integer phase mod 2^n."""
-    y = np.arange(1 << n, dtype=np.int64)
+def qft_amplitudes(n: int, x: int, indices=None) -> np.ndarray:
+    """Closed form of qft(n, x)'s output (complex128) at `indices` (default: every basis state),
+    with the phase x * rev(y) taken exactly as an integer mod 2^n."""
+    y = np.arange(1 << n, dtype=np.int64) if indices is None else np.asarray(indices, dtype=np.int64)
     rev = np.zeros_like(y)
     for b in range(n):
         rev |= ((y >> b) & 1) << (n - 1 - b)

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import pyquafu_amd as pq
+from oracle import dense
 from oracle.cpu import CpuState, sample_tree_on
 from pyquafu_amd import _native, workloads
 from pyquafu_amd.simulators.program import compile_circuit
@@ -332,3 +333,39 @@ def test_option_validation():
         dev.set_option("tile_bits", 14)  # 2^14 complex128 amplitudes exceed the LDS
     with pytest.raises(_native.EngineError):
         dev.set_option("no_such_option", 1)
+
+
+def test_gather_and_inner():
+    rng = np.random.Generator(np.random.PCG64(1800))
+    n = 12
+    a = rand_state(n, rng)
+    b = rand_state(n, rng)
+    da = _native.DeviceState(n, "c128")
+    db = _native.DeviceState(n, "c64")
+    da.load(a)
+    db.load(b)
+    idx = rng.integers(0, 1 << n, size=300)
+    assert np.array_equal(da.gather(idx), a[idx])
+    b64 = b.astype(np.complex64).astype(np.complex128)
+    assert np.abs(db.gather(idx) - b64[idx]).max() == 0.0
+    assert abs(da.inner(db) - np.vdot(a, b64)) < 1e-12
+    assert abs(db.inner(da) - np.vdot(b64, a)) < 1e-12
+    with pytest.raises(_native.EngineError):
+        da.gather([1 << n])
+    dc = _native.DeviceState(n - 1)
+    with pytest.raises(_native.EngineError):
+        da.inner(dc)
+    for d in (da, db, dc):
+        d.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-12), ("c64", 2e-6)])
+def test_qft_closed_form_22_qubits(dtype, tol):
+    n = 22
+    x = 0x2B5E31
+    dev = _native.DeviceState(n, dtype)
+    dev.apply(compile_circuit(workloads.qft(n, x)))
+    got = dev.store()
+    dev.close()
+    want = dense.qft_closed_form(n, x)
+    assert np.abs(got - want).max() < tol
