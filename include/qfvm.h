@@ -235,7 +235,7 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
 /*
  * The execution plan qfvm_apply would run for `prog` on an n-qubit state, as JSON (host only, no
  * device needed; no counterpart in the reference). Fused passes list their tile qubits and, per
- * 4-qubit group, the ops in execution order: {"kind", "g", "cmg", "cme", "m": [re, im] x 4}; other
+ * 4-qubit group, the ops in execution order: {"kind", "g", "cmg", "cme", "run", "m": [re, im] x 4}; other
  * items carry the gate. Lets the planner's reordering be checked without a GPU. Arguments of 0
  * pick the defaults, as for qfvm_set_option.
  */

@@ -23,8 +23,19 @@ enum FKind : int32_t {
     F_PHASE = 4,  // diag(d0, d1) on a qubit outside the group: g = that qubit's global position
     F_FLUSH = 5,  // apply the pending relabeling of group bit g to the data (mask in pxmask)
     F_GRAD = 6,   // gradient marker (stage groups only): g = GradGen, see FOpHost
-    F_RELABEL = 7 // host only: an X folded into the relabeling of bit g (listed, never uploaded)
+    F_RELABEL = 7, // host only: an X folded into the relabeling of bit g (listed, never uploaded)
+    F_DIAGRUN = 8  // device only: a run of diagonal ops applied at once (see below)
 };
+
+// Diagonal runs. Diagonal gates commute, so consecutive diagonal ops of a group (relabelings may
+// sit between them) act together as one factor per register member: member M of a row is
+// multiplied by the product of f_S over the subsets S of M's bits, where each op contributes
+// "terms" (S, lane condition, value). A diagonal on group bit j with group controls C gives
+// (S = C, d0) and (S = C + j, d1 / d0); with a pending relabel of bit j the two values are chosen
+// per lane by the relabel parity. A diagonal on a qubit outside the group gives (S = C, d0 or d1 by
+// that qubit's bit). Controls outside the group make a term conditional on the row's index.
+// Terms that are the same for every lane are multiplied into f_S on the host.
+constexpr int DIAGRUN_MIN = 8;  // shortest run worth merging
 
 // X gates without a control among the group bits move no data. An X on group bit j whose controls
 // outside the group are one qubit c (or none) flips bit j of the row's relabeling exactly where bit
@@ -52,6 +63,7 @@ struct FOpHost {
     int32_t target;     // F_GRAD: the generator's qubit (global position)
     int32_t slot;       // F_GRAD: output index
     uint64_t pxmask;    // FL_TARGET_PX / F_FLUSH: relabel parity mask of the op's bit g
+    int32_t run;        // 1 + index of the diagonal run the op belongs to in its group (0: none)
 };
 
 // A group is either a compute group (4 tile qubits and their gates) or, in the two-state reverse

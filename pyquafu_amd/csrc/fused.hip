@@ -15,6 +15,7 @@
 #include <cstring>
 
 #include "fused.h"
+#include "sincos.h"
 
 namespace qfvm {
 
@@ -38,6 +39,23 @@ struct PassDesc {
     int gop[FUSE_MAX_GROUPS + 1];                  // op ranges in the op array
     uint64_t gpxm[FUSE_MAX_GROUPS][FUSE_GROUP_BITS];  // relabel masks at each group's end
     uint32_t gpxc[FUSE_MAX_GROUPS];                   // relabel constants at each group's end
+};
+
+// Diagonal-run record (fused.h): 16 host factors f0, then for each subset S its term range
+// [tbeg[S], tbeg[S+1]) in the term array that follows the header.
+template <typename R>
+struct RunTerm {
+    uint64_t cl;    // index bits outside the group that must all be set
+    uint64_t pm;    // selector: v1 where parity(G0 & pm) ^ pc is 1, else v0
+    uint32_t pc;
+    int32_t angle;  // 1: unit-modulus values given as angles (v[0] = arg v0, v[1] = arg v1)
+    R v[4];         // v0 (re, im), v1 (re, im)
+};
+template <typename R>
+struct RunHead {
+    R f0[32];
+    int32_t tbeg[17];
+    int32_t pad[3];
 };
 
 // bit j of the row's relabeling: parity(G0 & mask) ^ constant
@@ -208,8 +226,51 @@ __device__ __forceinline__ void op_kind(V2<R> (&v)[16], const FOpDev<R>& o, bool
     }
 }
 
+template <typename R, int S, int M>
+__device__ __forceinline__ void run_mul(V2<R> (&v)[16], V2<R> f) {
+    if constexpr ((M & S) == S) v[M] = cmul<R>(f, v[M]);
+}
+
+// factor f_S of a diagonal run (host part times this lane's terms), applied to the members M ⊇ S
+template <typename R, int S, int... Ms>
+__device__ __forceinline__ void run_subset(V2<R> (&v)[16], const RunHead<R>* __restrict__ rh,
+                                           const RunTerm<R>* __restrict__ terms, uint32_t active, uint32_t angles,
+                                           uint64_t G0, std::integer_sequence<int, Ms...>) {
+    if (!((active >> S) & 1u)) return;
+    V2<R> f = mk<R>(rh->f0[2 * S], rh->f0[2 * S + 1]);
+    R th = 0;  // unit-modulus terms add angles; one sincos per subset at the end
+    const int t1 = rh->tbeg[S + 1];
+    for (int t = rh->tbeg[S]; t < t1; t++) {
+        const RunTerm<R>& tm = terms[t];
+        const bool cond = (G0 & tm.cl) == tm.cl;
+        const bool sel = px_bit(G0, tm.pm, tm.pc);
+        if (tm.angle) {
+            th += cond ? (sel ? tm.v[1] : tm.v[0]) : (R)0;
+        } else {
+            V2<R> val = csel<R>(sel, mk<R>(tm.v[2], tm.v[3]), mk<R>(tm.v[0], tm.v[1]));
+            val = csel<R>(cond, val, mk<R>(1, 0));
+            f = cmul<R>(f, val);
+        }
+    }
+    if ((angles >> S) & 1u) {
+        R sn, cs;
+        sin_cos(th, &sn, &cs);
+        f = cmul<R>(f, mk<R>(cs, sn));
+    }
+    (run_mul<R, S, Ms>(v, f), ...);
+}
+
+template <typename R, int... Ss>
+__device__ __forceinline__ void diag_run(V2<R> (&v)[16], const unsigned char* __restrict__ runs, const FOpDev<R>& o,
+                                         uint64_t G0, std::integer_sequence<int, Ss...>) {
+    const RunHead<R>* rh = reinterpret_cast<const RunHead<R>*>(runs + o.cm_ext);
+    const RunTerm<R>* terms = reinterpret_cast<const RunTerm<R>*>(rh + 1);
+    (run_subset<R, Ss>(v, rh, terms, (uint32_t)o.g, o.flags, G0, std::make_integer_sequence<int, 16>{}), ...);
+}
+
 template <typename R>
-__device__ __forceinline__ void apply_op(V2<R> (&v)[16], const FOpDev<R>& o, uint64_t G0) {
+__device__ __forceinline__ void apply_op(V2<R> (&v)[16], const FOpDev<R>& o, uint64_t G0,
+                                         const unsigned char* __restrict__ runs) {
     const bool ok = (G0 & o.cm_ext) == o.cm_ext;
     switch (o.kind) {
         case F_DENSE: op_kind<R, F_DENSE>(v, o, ok, G0); break;
@@ -231,6 +292,7 @@ __device__ __forceinline__ void apply_op(V2<R> (&v)[16], const FOpDev<R>& o, uin
             }
             break;
         }
+        case F_DIAGRUN: diag_run<R>(v, runs, o, G0, std::make_integer_sequence<int, 16>{}); break;
         default: {  // F_PHASE: one factor for the whole row, chosen by an index bit outside the group
             V2<R> d = csel<R>((G0 >> o.g) & 1ull, mk<R>(o.m[6], o.m[7]), mk<R>(o.m[0], o.m[1]));
             d = csel<R>(ok, d, mk<R>(1, 0));
@@ -307,6 +369,7 @@ __device__ void grad_stage(const V2<R>* phi, const V2<R>* lam, int T, uint64_t g
 template <typename R, int NS>
 __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<R>* __restrict__ psi1,
                                                        const PassDesc pd, const FOpDev<R>* __restrict__ ops,
+                                                       const unsigned char* __restrict__ runs,
                                                        double* __restrict__ partials, uint64_t tiles) {
     extern __shared__ unsigned char lds_raw[];
     V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
@@ -367,7 +430,7 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
                 V2<R>* tl = tile + st * T;
                 V2<R> v[16];
                 gather_seq<R>(v, tl, a, std::make_integer_sequence<int, 16>{});
-                for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, ops[oi], G0);
+                for (int oi = o_begin; oi < o_end; oi++) apply_op<R>(v, ops[oi], G0, runs);
                 const uint32_t pc = pd.gpxc[gi];
                 const int xo = swz((px_bit(G0, pd.gpxm[gi][0], pc) << s0) | (px_bit(G0, pd.gpxm[gi][1], pc >> 1) << s1) |
                                    (px_bit(G0, pd.gpxm[gi][2], pc >> 2) << s2) |
@@ -390,10 +453,91 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
 }
 
 // ------------------------------------------------------------------------------------------------
-// Appends the pass's ops to h and, for two-state passes, the output slot of each marker column to
-// slots (column c of the pass = its c-th marker).
+// Appends the record of the diagonal run `run` of group G to `runs` (16-byte aligned) and returns
+// the device op that applies it.
 template <typename R>
-static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R>>& h, std::vector<int>& slots) {
+static FOpDev<R> encode_run(const FGroup& G, int run, std::vector<unsigned char>& runs) {
+    struct HT {
+        int S;
+        uint64_t cl, pm;
+        uint32_t pc;
+        cd v0, v1;
+    };
+    cd f0[16];
+    for (int S = 0; S < 16; S++) f0[S] = 1.0;
+    std::vector<HT> terms;
+    auto add = [&](int S, uint64_t cl, uint64_t pm, uint32_t pc, cd v0, cd v1) {
+        if (cl == 0 && pm == 0)
+            f0[S] *= (pc & 1u) ? v1 : v0;
+        else if (!(v0 == cd(1.0, 0.0) && v1 == cd(1.0, 0.0)))
+            terms.push_back(HT{S, cl, pm, pc & 1u, v0, v1});
+    };
+    for (const FOpHost& o : G.ops) {
+        if (o.run != run) continue;
+        const cd d0 = o.m[0], d1 = o.m[3];
+        const int C = (int)o.cm_group;
+        if (o.kind == F_PHASE) {
+            add(C, o.cm_ext, 1ull << o.g, 0, d0, d1);
+        } else if (o.flags & FL_TARGET_PX) {
+            const uint32_t pc = (o.flags & FL_PX_CONST) ? 1u : 0u;
+            add(C, o.cm_ext, o.pxmask, pc, d0, d1);
+            add(C | (1 << o.g), o.cm_ext, o.pxmask, pc, d1 / d0, d0 / d1);
+        } else {
+            add(C, o.cm_ext, 0, 0, d0, d0);
+            add(C | (1 << o.g), o.cm_ext, 0, 0, d1 / d0, d1 / d0);
+        }
+    }
+    std::stable_sort(terms.begin(), terms.end(), [](const HT& a, const HT& b) { return a.S < b.S; });
+    RunHead<R> head{};
+    uint32_t active = 0, angles = 0;
+    auto unit = [](cd z) { return std::abs(std::abs(z) - 1.0) < 1e-14; };
+    for (int S = 0; S < 16; S++) {
+        head.f0[2 * S] = (R)f0[S].real();
+        head.f0[2 * S + 1] = (R)f0[S].imag();
+        if (f0[S] != cd(1.0, 0.0)) active |= 1u << S;
+    }
+    size_t t = 0;
+    for (int S = 0; S <= 16; S++) {
+        head.tbeg[S] = (int32_t)t;
+        while (S < 16 && t < terms.size() && terms[t].S == S) {
+            active |= 1u << S;
+            t++;
+        }
+    }
+    const size_t off = (runs.size() + 15) / 16 * 16;
+    runs.resize(off + sizeof(head) + terms.size() * sizeof(RunTerm<R>));
+    std::memcpy(runs.data() + off, &head, sizeof(head));
+    for (size_t k = 0; k < terms.size(); k++) {
+        RunTerm<R> rt{};
+        rt.cl = terms[k].cl;
+        rt.pm = terms[k].pm;
+        rt.pc = terms[k].pc;
+        if (unit(terms[k].v0) && unit(terms[k].v1)) {
+            rt.angle = 1;
+            rt.v[0] = (R)std::arg(terms[k].v0);
+            rt.v[1] = (R)std::arg(terms[k].v1);
+            angles |= 1u << terms[k].S;
+        } else {
+            rt.v[0] = (R)terms[k].v0.real();
+            rt.v[1] = (R)terms[k].v0.imag();
+            rt.v[2] = (R)terms[k].v1.real();
+            rt.v[3] = (R)terms[k].v1.imag();
+        }
+        std::memcpy(runs.data() + off + sizeof(head) + k * sizeof(rt), &rt, sizeof(rt));
+    }
+    FOpDev<R> d{};
+    d.kind = F_DIAGRUN;
+    d.g = (int32_t)active;
+    d.flags = angles;
+    d.cm_ext = off;
+    return d;
+}
+
+// Appends the pass's ops to h (diagonal-run records to runs) and, for two-state passes, the output
+// slot of each marker column to slots (column c of the pass = its c-th marker).
+template <typename R>
+static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R>>& h, std::vector<int>& slots,
+                            std::vector<unsigned char>& runs) {
     const int b = (int)p.Q.size();
     int L = 0;
     while (L < b && p.Q[L] == L) L++;
@@ -439,8 +583,16 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
         pd.gop[gi] = (int)h.size() - first;
         for (int j = 0; j < FUSE_GROUP_BITS; j++) pd.gpxm[gi][j] = G.pxm[j];
         pd.gpxc[gi] = G.pxc;
+        int done_run = 0;
         for (const FOpHost& o : G.ops) {
             if (o.kind == F_RELABEL) continue;  // folded into the parity masks
+            if (o.run > 0) {                    // runs are numbered in op order
+                if (o.run > done_run) {
+                    h.push_back(encode_run<R>(G, o.run, runs));
+                    done_run = o.run;
+                }
+                continue;
+            }
             FOpDev<R> d{};
             d.kind = o.kind;
             d.g = o.g;
@@ -468,11 +620,12 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
     std::vector<size_t> first;
     std::vector<int> slots_all;
     std::vector<size_t> slot_first, slot_count;
+    std::vector<unsigned char> runs;
     for (const PlanItem& it : plan) {
         if (!it.fused) continue;
         first.push_back(h.size());
         std::vector<int> slots;
-        descs.push_back(encode_pass<R>(s, it.pass, h, slots));
+        descs.push_back(encode_pass<R>(s, it.pass, h, slots, runs));
         if (NS == 1 && !slots.empty()) throw Error(QFVM_ERR_RUNTIME, "gradient marker in a one-state pass");
         slot_first.push_back(slots_all.size());
         slot_count.push_back(slots.size());
@@ -480,18 +633,22 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
     }
     FOpDev<R>* dops = nullptr;
     int* dslots = nullptr;
+    const unsigned char* druns = nullptr;
     if (!h.empty()) {
         const size_t obytes = h.size() * sizeof(FOpDev<R>);
         const size_t soff = (obytes + 255) / 256 * 256;
-        const size_t bytes = soff + slots_all.size() * sizeof(int);
+        const size_t roff = (soff + slots_all.size() * sizeof(int) + 255) / 256 * 256;
+        const size_t bytes = roff + runs.size();
         QFVM_HIP(hipStreamSynchronize(s.stream));  // the pinned stage may still feed an earlier copy
         unsigned char* dev = reinterpret_cast<unsigned char*>(s.ensure_ops(bytes));
         unsigned char* stage = reinterpret_cast<unsigned char*>(s.ensure_pinned(bytes));
         std::memcpy(stage, h.data(), obytes);
         if (!slots_all.empty()) std::memcpy(stage + soff, slots_all.data(), slots_all.size() * sizeof(int));
+        if (!runs.empty()) std::memcpy(stage + roff, runs.data(), runs.size());
         QFVM_HIP(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, s.stream));
         dops = reinterpret_cast<FOpDev<R>*>(dev);
         dslots = reinterpret_cast<int*>(dev + soff);
+        druns = dev + roff;
     }
     V2<R>* psi = reinterpret_cast<V2<R>*>(s.psi);
     V2<R>* psi1 = reinterpret_cast<V2<R>*>(lam);
@@ -528,7 +685,7 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         {
             KernelScope ks(s, NS == 1 ? "fused_pass" : "fused_pass2", 2.0 * NS * (double)s.size * sizeof(V2<R>));
             hipLaunchKernelGGL((k_fused_pass<R, NS>), dim3((unsigned)tiles), dim3(threads), lds, s.stream, psi, psi1,
-                               pd, ops, partials, tiles);
+                               pd, ops, druns, partials, tiles);
             QFVM_LAUNCHED();
         }
         if (cols > 0) launch_column_sums(s, partials, tiles, cols, slot_of, grads_dev);

@@ -108,6 +108,33 @@ FOpHost convert(const Gate& g, const GInfo& gi, const int S[4]) {
     return o;
 }
 
+// Groups consecutive diagonal ops (relabelings may sit between them) into runs of at least
+// DIAGRUN_MIN ops (fused.h). An op joins only if its terms can be formed: d0 != 0, and d1 != 0 when
+// its target bit carries a pending relabeling.
+void mark_diagonal_runs(FGroup& fg) {
+    int next = 1;
+    std::vector<int> cur;
+    auto close = [&]() {
+        if ((int)cur.size() >= DIAGRUN_MIN) {
+            for (int k : cur) fg.ops[k].run = next;
+            next++;
+        }
+        cur.clear();
+    };
+    for (int k = 0; k < (int)fg.ops.size(); k++) {
+        const FOpHost& o = fg.ops[k];
+        if (o.kind == F_RELABEL) continue;
+        const bool diag = o.kind == F_DIAG || o.kind == F_PHASE;
+        const bool ok = diag && o.m[0] != cd(0.0, 0.0) &&
+                        !((o.flags & FL_TARGET_PX) && o.m[3] == cd(0.0, 0.0));
+        if (ok)
+            cur.push_back(k);
+        else
+            close();
+    }
+    close();
+}
+
 }  // namespace
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg) {
@@ -256,6 +283,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             }
             for (int j = 0; j < FUSE_GROUP_BITS; j++) fg.pxm[j] = M[j];
             fg.pxc = C;
+            mark_diagonal_runs(fg);
             it.pass.groups.push_back(std::move(fg));
         }
         plan.push_back(std::move(it));

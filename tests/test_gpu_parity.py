@@ -369,3 +369,50 @@ def test_qft_closed_form_22_qubits(dtype, tol):
     dev.close()
     want = dense.qft_closed_form(n, x)
     assert np.abs(got - want).max() < tol
+
+
+def diagonal_heavy_program(n: int, n_ops: int, rng):
+    """Mostly diagonal gates (phases, controlled phases, diagonal matrices with controls anywhere)
+    with X gates (relabelings) and a few dense gates between them: long diagonal runs."""
+    from pyquafu_amd.simulators.program import CP, CX, CZ, GENERAL, P, RZ, RZZ, S, T, X, ProgramBuilder
+
+    b = ProgramBuilder()
+    for _ in range(n_ops):
+        k = int(rng.integers(0, 10))
+        if k < 2:
+            b.add(int(rng.choice([S, T, P, RZ])), [int(rng.integers(n))], param=float(rng.uniform(-7, 7)))
+        elif k < 5:
+            q = rng.choice(n, 2, replace=False)
+            b.add(int(rng.choice([CP, CZ, RZZ])), [int(x) for x in q], n_ctrl=1, param=float(rng.uniform(-7, 7)))
+        elif k < 7:
+            nc = int(rng.integers(0, 3))
+            q = rng.choice(n, nc + 1, replace=False)
+            d = np.exp(1j * rng.uniform(-4, 4, 2))
+            b.add(GENERAL, [int(x) for x in q], n_ctrl=nc, matrix=np.diag(d))
+        elif k < 9:
+            q = rng.choice(n, 2, replace=False)
+            if rng.integers(2):
+                b.add(X, [int(q[0])])
+            else:
+                b.add(CX, [int(x) for x in q], n_ctrl=1)
+        else:
+            from tests.progs import random_unitary
+            b.add(GENERAL, [int(rng.integers(n))], matrix=random_unitary(2, rng))
+    return b.program(n_qubits=n, n_cbits=n)
+
+
+@pytest.mark.parametrize("n,dtype", [(9, "c128"), (13, "c128"), (16, "c128"), (13, "c64")])
+def test_diagonal_runs_match_oracle(n, dtype):
+    rng = np.random.Generator(np.random.PCG64(1900 + n))
+    prog = diagonal_heavy_program(n, 400, rng)
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    cpu.apply(prog)
+    dev = _native.DeviceState(n, dtype)
+    dev.load(psi0)
+    dev.apply(prog)
+    got = dev.store()
+    dev.close()
+    tol = 1e-12 if dtype == "c128" else 2e-5
+    assert np.abs(got - cpu.store()).max() < tol
