@@ -243,6 +243,42 @@ def qft_leg(n: int, shots: int, samples: int = 100000):
     return out
 
 
+def batch_leg(count: int, workers: int):
+    """Batched uneven workload: `count` seeded circuits (PCG64 seed 20250004), n ~ U{10..22}, depth ~
+    U{4..16} layers of RY and RZ on every qubit plus a CZ ring, complex64; each evaluates <sum_i Z_i>
+    and its full adjoint gradient. `workers` host threads drive one engine handle (one HIP stream)
+    per qubit count each; tile shapes from the autotuner. Compilation and tuning happen before
+    the timed run; one untimed run warms the plans."""
+    from pyquafu_amd import workloads
+    from pyquafu_amd.batch import Autotuner, BatchRunner, compile_item
+
+    t0 = time.perf_counter()
+    circuits = workloads.batch_circuits(count=count)
+    items = [compile_item(qc, workloads.z_sum(qc.num)) for qc, _ in circuits]
+    t_compile = time.perf_counter() - t0
+    tuner = Autotuner()
+    runner = BatchRunner(dtype="c64", workers=workers, tuner=tuner)
+    t0 = time.perf_counter()
+    runner.warm([it.n for it in items])
+    t_warm = time.perf_counter() - t0
+    runner.run(items)
+    results, wall = runner.run(items)
+    runner.close()
+    lat = np.array([r.latency_s for r in results]) * 1e3
+    return {
+        "workload": f"{count} circuits, n 10-22, depth 4-16 (RY+RZ layers, CZ ring), c64, <sum Z> + adjoint gradient",
+        "circuits_per_s": round(count / wall, 1),
+        "wall_ms": round(wall * 1e3, 2),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "workers": workers,
+        "parameters": int(sum(len(it.ops) for it in items)),
+        "host_compile_s": round(t_compile, 2),
+        "tune_and_warm_s": round(t_warm, 2),
+        "tiles": {str(k[0]): list(v) for k, v in sorted(tuner.choice.items())},
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +299,9 @@ def main(argv=None) -> int:
     ap.add_argument("--adjoint-n", type=int, default=24)
     ap.add_argument("--adjoint-layers", type=int, default=8)
     ap.add_argument("--no-qft", action="store_true", help="skip the 33-qubit QFT precision leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-circuit leg")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-workers", type=int, default=4)
     ap.add_argument("--qft-n", type=int, default=33)
     ap.add_argument("--qft-shots", type=int, default=1000000)
     args = ap.parse_args(argv)
@@ -360,6 +399,11 @@ def main(argv=None) -> int:
         out["adjoint"] = adjoint_leg(args.adjoint_n, args.adjoint_layers)
     else:
         out["adjoint"] = None
+    if not args.no_batch:
+        print(f"[bench] batch leg: {args.batch_count} circuits", file=sys.stderr, flush=True)
+        out["batch"] = batch_leg(args.batch_count, args.batch_workers)
+    else:
+        out["batch"] = None
     if not args.no_qft:
         print(f"[bench] QFT precision leg: {args.qft_n}q, c64 and c128", file=sys.stderr, flush=True)
         out["precision"] = qft_leg(args.qft_n, args.qft_shots)

@@ -165,6 +165,24 @@ def _coeff_array(coeffs, n: int) -> np.ndarray:
     return out
 
 
+class AdjointCall:
+    """The C arguments of one qfvm_adjoint call, built once so that repeated calls (batches,
+    training loops) skip the conversion."""
+
+    def __init__(self, prog: Program, grad_ops, grad_gens: str, terms, coeffs):
+        self.cp, self._keep = as_cprogram(prog)
+        self._ops = np.ascontiguousarray(grad_ops, dtype=np.int64)
+        if len(grad_gens) != self._ops.size:
+            raise ValueError("one generator letter per differentiated op")
+        self._gens = np.frombuffer(grad_gens.encode() + b"\0", dtype=np.uint8)
+        self._terms = pauli_arrays(terms)
+        c, p, o, l_ = self._terms
+        self._cf = _coeff_array(coeffs, len(o))
+        self.n_grads = int(self._ops.size)
+        self.args = (self._ops.ctypes.data, self._gens.ctypes.data, self.n_grads, c.ctypes.data, p.ctypes.data,
+                     o.ctypes.data, l_.ctypes.data, self._cf.ctypes.data, len(o))
+
+
 class DeviceState:
     """One statevector resident in HBM, owned by the engine."""
 
@@ -243,20 +261,14 @@ class DeviceState:
 
     def adjoint(self, prog: Program, grad_ops, grad_gens: str, terms, coeffs):
         """Reverse-mode gradient of <H> after `prog` (qfvm_adjoint). Returns (grads, expectation)."""
-        cp, keep = as_cprogram(prog)
-        ops = np.ascontiguousarray(grad_ops, dtype=np.int64)
-        gens = np.frombuffer(grad_gens.encode() + b"\0", dtype=np.uint8)
-        if len(grad_gens) != ops.size:
-            raise ValueError("one generator letter per differentiated op")
-        c, p, o, l_ = pauli_arrays(terms)
-        cf = _coeff_array(coeffs, len(o))
-        grads = np.zeros(max(ops.size, 1), dtype=np.float64)
+        return self.adjoint_prepared(AdjointCall(prog, grad_ops, grad_gens, terms, coeffs))
+
+    def adjoint_prepared(self, call: "AdjointCall"):
+        """qfvm_adjoint with arguments converted once (see AdjointCall)."""
+        grads = np.zeros(max(call.n_grads, 1), dtype=np.float64)
         e = ctypes.c_double(0.0)
-        check(lib().qfvm_adjoint(self._h, ctypes.byref(cp), ops.ctypes.data, gens.ctypes.data, ops.size,
-                                 c.ctypes.data, p.ctypes.data, o.ctypes.data, l_.ctypes.data, cf.ctypes.data, len(o),
-                                 grads.ctypes.data, ctypes.byref(e)))
-        del keep
-        return grads[: ops.size], e.value
+        check(lib().qfvm_adjoint(self._h, ctypes.byref(call.cp), *call.args, grads.ctypes.data, ctypes.byref(e)))
+        return grads[: call.n_grads], e.value
 
     def gather(self, indices) -> np.ndarray:
         """Amplitudes at `indices` (complex128)."""

@@ -398,6 +398,43 @@ void* State::ensure_aux(size_t bytes) {
     return aux_buf;
 }
 
+unsigned char* State::stage_host(int k, size_t bytes) {
+    Stage& st = stages[k];
+    if (st.done) QFVM_HIP(hipEventSynchronize(st.done));
+    if (bytes > st.host_bytes) {
+        if (st.host) QFVM_HIP(hipHostFree(st.host));
+        st.host = nullptr;
+        st.host_bytes = 0;
+        const size_t want = std::max<size_t>(bytes, 1 << 16);
+        QFVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.host), want, hipHostMallocDefault));
+        st.host_bytes = want;
+    }
+    return st.host;
+}
+
+unsigned char* State::stage_dev(int k, size_t bytes) {
+    Stage& st = stages[k];
+    if (bytes > st.dev_bytes) {
+        if (st.dev) {
+            QFVM_HIP(hipStreamSynchronize(stream));  // queued kernels may still read the old area
+            QFVM_HIP(hipFree(st.dev));
+        }
+        st.dev = nullptr;
+        st.dev_bytes = 0;
+        const size_t want = std::max<size_t>(bytes, 1 << 16);
+        QFVM_HIP(hipMalloc(reinterpret_cast<void**>(&st.dev), want));
+        st.dev_bytes = want;
+    }
+    return st.dev;
+}
+
+void State::stage_upload(int k, size_t bytes) {
+    Stage& st = stages[k];
+    if (!st.done) QFVM_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    QFVM_HIP(hipMemcpyAsync(st.dev, st.host, bytes, hipMemcpyHostToDevice, stream));
+    QFVM_HIP(hipEventRecord(st.done, stream));
+}
+
 void* State::ensure_psi2() {
     if (!psi2) QFVM_HIP(hipMalloc(&psi2, size * amp_bytes()));
     return psi2;
@@ -410,6 +447,11 @@ static void destroy_state(State* s) {
     s->resolve_events();
     if (s->psi) hipFree(s->psi);
     if (s->psi2) hipFree(s->psi2);
+    for (Stage& st : s->stages) {
+        if (st.host) hipHostFree(st.host);
+        if (st.dev) hipFree(st.dev);
+        if (st.done) hipEventDestroy(st.done);
+    }
     if (s->aux_buf) hipFree(s->aux_buf);
     if (s->ops_buf) hipFree(s->ops_buf);
     if (s->scratch) hipFree(s->scratch);
@@ -607,17 +649,17 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
             marker_of[(size_t)k] = (int)g;
             begin = std::min(begin, k);
         }
+        // everything below is queued on the stream; the host plans the reverse sweep while the
+        // forward passes run, and waits once at the end. Results: grads[0 .. n), <H> at [n].
+        const size_t rbytes = ((size_t)n_grads + 1) * sizeof(double);
+        double* rdev = reinterpret_cast<double*>(s->ensure_aux(rbytes));
+        QFVM_HIP(hipMemsetAsync(rdev, 0, rbytes, s->stream));
         apply_gates(*s, fwd);
         void* lam = s->ensure_psi2();
-        const double e = H.groups.empty() ? 0.0 : apply_hamiltonian(*s, H, lam);
-        if (H.groups.empty()) QFVM_HIP(hipMemsetAsync(lam, 0, s->size * s->amp_bytes(), s->stream));
-        if (expectation) *expectation = e;
-        if (n_grads == 0) {
-            QFVM_HIP(hipStreamSynchronize(s->stream));
-            return;
-        }
-        double* gdev = reinterpret_cast<double*>(s->ensure_aux((size_t)n_grads * sizeof(double)));
-        QFVM_HIP(hipMemsetAsync(gdev, 0, (size_t)n_grads * sizeof(double), s->stream));
+        if (H.groups.empty())
+            QFVM_HIP(hipMemsetAsync(lam, 0, s->size * s->amp_bytes(), s->stream));
+        else
+            apply_hamiltonian(*s, H, lam, rdev + n_grads);
         // reverse sweep (gradiant.py:160-172): the marker of op k sits before U_k^dagger, where both
         // states are taken just after U_k
         std::vector<Gate> rev;
@@ -626,11 +668,12 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
             const auto r = range[(size_t)k];
             for (size_t j = r.second; j > r.first; j--) rev.push_back(dagger(fwd[j - 1]));
         }
-        apply_gates2(*s, rev, lam, gdev);
-        double* hbuf = s->ensure_pinned((size_t)n_grads * sizeof(double));
-        QFVM_HIP(hipMemcpyAsync(hbuf, gdev, (size_t)n_grads * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        if (n_grads > 0) apply_gates2(*s, rev, lam, rdev);
+        double* hbuf = reinterpret_cast<double*>(s->stage_host(STAGE_RESULT, rbytes));
+        QFVM_HIP(hipMemcpyAsync(hbuf, rdev, rbytes, hipMemcpyDeviceToHost, s->stream));
         QFVM_HIP(hipStreamSynchronize(s->stream));
-        std::memcpy(grads, hbuf, (size_t)n_grads * sizeof(double));
+        if (n_grads > 0) std::memcpy(grads, hbuf, (size_t)n_grads * sizeof(double));
+        if (expectation) *expectation = hbuf[n_grads];
     });
 }
 

@@ -104,6 +104,18 @@ struct KStat {
     double bytes = 0.0;   // summed algorithmic bytes
 };
 
+// Upload slots: a pinned host area and a device area per purpose. A slot's host area is reused
+// once the previous copy out of it has finished (an event), so uploads need no stream sync; the
+// device area is only rewritten by later copies on the same stream, after the kernels that read it.
+enum StageSlot : int { STAGE_PLAN = 0, STAGE_PLAN2 = 1, STAGE_HAM = 2, STAGE_RESULT = 3, STAGE_COUNT = 4 };
+struct Stage {
+    unsigned char* host = nullptr;
+    size_t host_bytes = 0;
+    unsigned char* dev = nullptr;
+    size_t dev_bytes = 0;
+    hipEvent_t done = nullptr;
+};
+
 // per-handle state
 struct State {
     int n = 0;
@@ -122,6 +134,7 @@ struct State {
     void* aux_buf = nullptr;  // Hamiltonian terms and gradient outputs
     size_t aux_bytes = 0;
     void* psi2 = nullptr;     // second state of the reverse sweep (allocated on first use)
+    Stage stages[STAGE_COUNT];
     // execution options (qfvm_set_option); tile_bits / low_bits of 0 pick the defaults
     int fusion = 1;
     int tile_bits = 0;
@@ -142,6 +155,9 @@ struct State {
     double* ensure_pinned(size_t bytes);
     void* ensure_ops(size_t bytes);
     void* ensure_aux(size_t bytes);
+    unsigned char* stage_host(int slot, size_t bytes);  // waits for the slot's previous copy
+    unsigned char* stage_dev(int slot, size_t bytes);
+    void stage_upload(int slot, size_t bytes);           // host area -> device area, async
     void* ensure_psi2();
     int stat_index(const char* name);
     void resolve_events();  // fold finished event pairs into kstats (call after a stream sync)
@@ -193,8 +209,9 @@ void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double
 void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale);
 void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b);
 void sample_tree(State& s, const double* host_u, int64_t shots, uint64_t* host_out);
-// Re <psi|H|psi>; when `out` is not null, H psi is written there (a buffer shaped like the state)
-double apply_hamiltonian(State& s, const Hamiltonian& H, void* out);
+// Re <psi|H|psi>; when `out` is not null, H psi is written there (a buffer shaped like the state).
+// With dev_result, the value is left in that device double instead (no sync; returns 0).
+double apply_hamiltonian(State& s, const Hamiltonian& H, void* out, double* dev_result = nullptr);
 // grads_dev[g.slot] = Im <lam| P (controls projected) |psi> for one G_GRAD marker, over the whole state
 void launch_grad_marker(State& s, const Gate& g, const void* lam, double* grads_dev);
 // amplitudes at host-given indices (out: re, im pairs) and <a|b> of two states of one size

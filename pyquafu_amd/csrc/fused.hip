@@ -639,13 +639,15 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         const size_t soff = (obytes + 255) / 256 * 256;
         const size_t roff = (soff + slots_all.size() * sizeof(int) + 255) / 256 * 256;
         const size_t bytes = roff + runs.size();
-        QFVM_HIP(hipStreamSynchronize(s.stream));  // the pinned stage may still feed an earlier copy
-        unsigned char* dev = reinterpret_cast<unsigned char*>(s.ensure_ops(bytes));
-        unsigned char* stage = reinterpret_cast<unsigned char*>(s.ensure_pinned(bytes));
+        // one-state plans and the reverse sweep upload through different slots, so the adjoint's
+        // two plans can be in flight together
+        const int slot = NS == 2 ? STAGE_PLAN2 : STAGE_PLAN;
+        unsigned char* stage = s.stage_host(slot, bytes);
+        unsigned char* dev = s.stage_dev(slot, bytes);
         std::memcpy(stage, h.data(), obytes);
         if (!slots_all.empty()) std::memcpy(stage + soff, slots_all.data(), slots_all.size() * sizeof(int));
         if (!runs.empty()) std::memcpy(stage + roff, runs.data(), runs.size());
-        QFVM_HIP(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, s.stream));
+        s.stage_upload(slot, bytes);
         dops = reinterpret_cast<FOpDev<R>*>(dev);
         dslots = reinterpret_cast<int*>(dev + soff);
         druns = dev + roff;

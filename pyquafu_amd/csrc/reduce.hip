@@ -342,16 +342,15 @@ __global__ void __launch_bounds__(RED_BLOCK) k_column_sums(const double* __restr
     if (threadIdx.x == 0) grads[slot_of[blockIdx.x]] = acc;
 }
 
-double apply_hamiltonian(State& s, const Hamiltonian& H, void* out) {
+double apply_hamiltonian(State& s, const Hamiltonian& H, void* out, double* dev_result) {
     const size_t gbytes = H.groups.size() * sizeof(HamGroup), tbytes = H.terms.size() * sizeof(HamTerm);
     const size_t toff = (gbytes + 255) / 256 * 256;
-    unsigned char* dev = reinterpret_cast<unsigned char*>(s.ensure_aux(toff + tbytes + 256));
+    unsigned char* stage = s.stage_host(STAGE_HAM, toff + tbytes + 256);
+    unsigned char* dev = s.stage_dev(STAGE_HAM, toff + tbytes + 256);
     if (!H.groups.empty()) {
-        QFVM_HIP(hipStreamSynchronize(s.stream));  // the pinned stage may still feed an earlier copy
-        unsigned char* stage = reinterpret_cast<unsigned char*>(s.ensure_pinned(toff + tbytes));
         std::memcpy(stage, H.groups.data(), gbytes);
         std::memcpy(stage + toff, H.terms.data(), tbytes);
-        QFVM_HIP(hipMemcpyAsync(dev, stage, toff + tbytes, hipMemcpyHostToDevice, s.stream));
+        s.stage_upload(STAGE_HAM, toff + tbytes);
     }
     double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
     const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(s.size, RED_BLOCK));
@@ -367,6 +366,11 @@ double apply_hamiltonian(State& s, const Hamiltonian& H, void* out) {
                            reinterpret_cast<const float2*>(s.psi), reinterpret_cast<float2*>(out), s.size, g,
                            (int)H.groups.size(), t, partial);
     QFVM_LAUNCHED();
+    if (dev_result) {  // leave the sum on the device; the caller reads it after its own sync
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s.stream, partial, (int)grid, dev_result);
+        QFVM_LAUNCHED();
+        return 0.0;
+    }
     return finish_sum(s, partial, grid);
 }
 

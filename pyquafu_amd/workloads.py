@@ -64,6 +64,36 @@ def hea_ansatz(n: int, layers: int, seed: int = 20250003, parametrised: bool = T
     return qc, values
 
 
+def batch_circuits(count: int = 256, seed: int = 20250004, n_range=(10, 22), depth_range=(4, 16)):
+    """Uneven batch: per circuit, n ~ U{n_range} and depth ~ U{depth_range} drawn first (in that
+    order), then 2 n d parameters ~ U[-pi, pi]; layers of RY and RZ on every qubit and a CZ ring
+    (hea_ansatz's layout). Returns [(circuit, values)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        d = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        values = rng.uniform(-np.pi, np.pi, size=2 * n * d)
+        qc = QuantumCircuit(n)
+        k = 0
+        for _ in range(d):
+            for q in range(n):
+                qc << RYGate(q, Parameter(f"t{k}", values[k]))
+                k += 1
+            for q in range(n):
+                qc << RZGate(q, Parameter(f"t{k}", values[k]))
+                k += 1
+            for i in range(n):
+                qc << CZGate(i, (i + 1) % n)
+        out.append((qc, values))
+    return out
+
+
+def z_sum(n: int) -> Hamiltonian:
+    """sum_i Z_i."""
+    return Hamiltonian([PauliOp(f"Z{i}") for i in range(n)])
+
+
 def tfim_hamiltonian(n: int, h: float = 0.7) -> Hamiltonian:
     terms = [PauliOp(f"Z{i} Z{(i + 1) % n}") for i in range(n)]
     terms += [PauliOp(f"X{i}", h) for i in range(n)]

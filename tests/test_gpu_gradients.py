@@ -237,3 +237,23 @@ def test_execute_circuits_jacobians_agree():
             want = [og.expectation(psi, n, [("Z", [i])], [1.0]) for i in range(n)]
             assert np.abs(out[r].detach().numpy() - want).max() < 1e-12
     assert np.abs(grads["internal"] - grads["adjoint"]).max() < 1e-9
+
+
+def test_batch_runner_matches_single_calls():
+    from pyquafu_amd.batch import Autotuner, BatchRunner, compile_item
+
+    circuits = workloads.batch_circuits(count=24, seed=77, n_range=(6, 12), depth_range=(2, 5))
+    items = [compile_item(qc, workloads.z_sum(qc.num)) for qc, _ in circuits]
+    runner = BatchRunner(dtype="c128", workers=4, tuner=Autotuner(probe_layers=2))
+    runner.warm([it.n for it in items])
+    results, wall = runner.run(items)
+    runner.close()
+    assert len(results) == len(items) and wall > 0
+    for it, r in zip(items, results):
+        dev = _native.DeviceState(it.n)
+        dev.init_zero()
+        g, e = dev.adjoint(it.prog, it.ops, it.gens, it.terms, it.coeffs)
+        dev.close()
+        want, e_want = og.adjoint(it.prog, it.n, it.ops, it.gens, it.terms, it.coeffs)
+        assert abs(r.expectation - e_want) < 1e-11 and abs(e - e_want) < 1e-11
+        assert np.abs(r.grads - want).max() < 1e-10 and np.abs(g - want).max() < 1e-10
