@@ -68,7 +68,10 @@ typedef enum qfvm_opcode {
     QFVM_OP_MEASURE = 19, /* qubits -> cbits, projective, consumes one uniform */
     QFVM_OP_RESET = 20,   /* qubits -> |0>, consumes one uniform */
     QFVM_OP_CIF = 21,     /* if creg[cbits] (big-endian) == cond: run the next `body` ops */
-    QFVM_OP_COUNT = 22
+    QFVM_OP_KRAUS = 22,   /* noise: `body` one-qubit Kraus matrices at m_off; one uniform */
+    QFVM_OP_UCHAN = 23,   /* noise: `body` one-qubit unitaries at m_off, then their probabilities
+                             (as re parts of `body` complex entries); one uniform */
+    QFVM_OP_COUNT = 24
 } qfvm_opcode;
 
 /*
@@ -138,7 +141,13 @@ int qfvm_apply(qfvm_handle h, const qfvm_program* prog);
  * Run a program with mid-circuit semantics: MEASURE collapses (apply_measure,
  * statevector.hpp:1342-1356), RESET projects to |0> (apply_reset, :1358-1362), CIF tests the
  * classical register (check_cif, :186-193). Each MEASURE / RESET consumes the next entry of
- * `uniforms` (values in [0,1)) in place of the reference's std::random_device draw. When
+ * `uniforms` (values in [0,1)) in place of the reference's std::random_device draw. KRAUS and
+ * UCHAN (noise channels, NoiseSVSimulator, simulator.py:148-223) also take one uniform u each:
+ * KRAUS applies K_k / |K_k psi| for the first k with u < sum_{j<=k} |K_j psi| / |psi| and leaves
+ * the state unchanged when there is none (SVSimulator._apply_op, simulator.py:59-74; the ratio of
+ * norms, not of squared norms, is the reference's rule); UCHAN applies unitary k, the first with
+ * u < cdf[k], cdf = cumulative probabilities divided by their total (numpy's choice,
+ * noise.py:64-68). When
  * skip_measure != 0, MEASURE ops are skipped (final-measure mode, simulator.hpp:119-122).
  * creg has n_creg entries (0/1), zeroed by the caller.
  */

@@ -16,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 
 # op codes of include/qfvm.h
-GENERAL, MULTI, X, Y, Z, H, S, SDG, T, TDG, P, RX, RY, RZ, CX, CZ, CP, CCX, RZZ, MEASURE, RESET, CIF = range(22)
+GENERAL, MULTI, X, Y, Z, H, S, SDG, T, TDG, P, RX, RY, RZ, CX, CZ, CP, CCX, RZZ, MEASURE, RESET, CIF, KRAUS, UCHAN = range(24)
 
 I2 = np.eye(2, dtype=complex)
 XM = np.array([[0, 1], [1, 0]], dtype=complex)
@@ -173,6 +173,34 @@ class DenseRunner:
         ctrls, targs, m = self.gate_parts(op)
         self.psi = apply_unitary(self.psi, self.n, ctrls, targs, m)
 
+    def channel(self, op):
+        """Noise channel on one qubit, with the next uniform u. Unitary channel: numpy's choice rule
+        (cumulative probabilities over their total, first entry above u; noise.py:64-68). Kraus
+        channel: SVSimulator._apply_op (simulator.py:59-74): K_k psi for k = 0, 1, ...; running sum
+        of |K_k psi| / |psi|; the first k with u below it gives K_k psi / |K_k psi|, none leaves psi."""
+        q = self._q(op)[0]
+        nk = int(op["body"])
+        a = self.p.mat[op["m_off"]: op["m_off"] + 8 * nk]
+        mats = (a[0::2] + 1j * a[1::2]).reshape(nk, 2, 2)
+        u = self.u[self.ui]
+        self.ui += 1
+        if int(op["code"]) == UCHAN:
+            probs = self.p.mat[op["m_off"] + 8 * nk: op["m_off"] + 10 * nk: 2]
+            cdf = np.cumsum(probs)
+            cdf = cdf / cdf[-1]
+            k = int(np.searchsorted(cdf, u, side="right"))
+            self.psi = apply_unitary(self.psi, self.n, [], [q], mats[min(k, nk - 1)])
+            return
+        norm0 = np.linalg.norm(self.psi)
+        s = 0.0
+        for k in range(nk):
+            t = apply_unitary(self.psi, self.n, [], [q], mats[k])
+            norm1 = np.linalg.norm(t)
+            s += norm1 / norm0
+            if u < s:
+                self.psi = t / norm1
+                return
+
     def exec(self, begin: int, end: int) -> None:
         k = begin
         ops = self.p.ops
@@ -197,6 +225,9 @@ class DenseRunner:
                 m = discrete_pick(pr, self.u[self.ui])
                 self.ui += 1
                 self.psi = project(self.psi, self.n, q, 0, m, pr[m])
+            elif c in (KRAUS, UCHAN):
+                k += 1
+                self.channel(op)
             elif c == CIF:
                 cb = self._c(op)
                 out = 0

@@ -16,6 +16,7 @@ import numpy as np
 
 from ..elements import element_gates as qeg
 from ..elements.instruction import Barrier, Cif, Delay, Instruction, Measure, Reset
+from ..elements.noise import KrausChannel, UnitaryChannel
 from ..elements.parameters import Parameter, ParameterExpression, ParameterType
 from ..elements.quantum_gate import CircuitWrapper, ControlledGate, QuantumGate
 from ..exceptions import CircuitError
@@ -109,15 +110,48 @@ class QuantumCircuit:
         self.gates.append(gate)
 
     def add_ins(self, ins: Instruction):
-        if isinstance(ins, (QuantumGate, Delay, Barrier)):
+        if isinstance(ins, (QuantumGate, Delay, Barrier, KrausChannel)):
             self.add_gate(ins)
         self.instructions.append(ins)
+
+    NOISE_CHANNELS = ("bitflip", "dephasing", "depolarizing", "ampdamping")
+
+    def add_noise(self, channel: str, channel_args, qubits: Optional[List[int]] = None,
+                  gates: Optional[List[str]] = None, checkgates: bool = True):
+        """After every gate that acts on one of `qubits` (and whose lowercased name is in `gates`,
+        when given), insert `channel(q, *channel_args)` for each of its positions q in `qubits`
+        (quantum_circuit.py:181-245). As there, an empty `qubits` adds nothing."""
+        qubits = [] if qubits is None else list(qubits)
+        gates = [] if gates is None else list(gates)
+        if channel not in self.NOISE_CHANNELS:
+            raise ValueError("Invalid channel name")
+        if checkgates:
+            for g in gates:
+                if g not in QuantumGate.gate_classes:
+                    raise ValueError("Invalid gate name")
+        cls = Instruction.ins_classes[channel]
+        instructions, kept = [], []
+        for op in self.instructions:
+            instructions.append(op)
+            if isinstance(op, (QuantumGate, Delay, Barrier, KrausChannel)):
+                kept.append(op)
+            if not isinstance(op, QuantumGate):
+                continue
+            if gates and op.name.lower() not in gates:
+                continue
+            for q in op.pos:
+                if q in qubits:
+                    instructions.append(cls(q, *channel_args))
+                    kept.append(cls(q, *channel_args))
+        self.instructions = instructions
+        self._gates = kept
+        return self
 
     @property
     def noised(self):
         if self._has_wrap:
             self.unwrap()
-        return False
+        return any(isinstance(op, KrausChannel) for op in self.instructions)
 
     # ------------------------------------------------------------------ parameters
     def get_parameter_grads(self):
@@ -276,11 +310,11 @@ class QuantumCircuit:
             if isinstance(op, CircuitWrapper):
                 for op_ in op.circuit.unwrap().instructions:
                     instructions.append(op_)
-                    if isinstance(op_, (QuantumGate, Delay, Barrier)):
+                    if isinstance(op_, (QuantumGate, Delay, Barrier, KrausChannel)):
                         gates.append(op_)
             else:
                 instructions.append(op)
-                if isinstance(op, (QuantumGate, Delay, Barrier)):
+                if isinstance(op, (QuantumGate, Delay, Barrier, KrausChannel)):
                     gates.append(op)
         self.instructions = instructions
         self._gates = gates

@@ -299,6 +299,54 @@ struct Runner {
             for (size_t j = 0; j < q.size(); j++) creg[(*c)[j]] = (int32_t)((m >> j) & 1);
     }
 
+    // a noise channel on one qubit (include/qfvm.h: QFVM_OP_KRAUS / QFVM_OP_UCHAN)
+    void channel(const qfvm_op& op) {
+        std::vector<int> q = op_list(p, op.q_off, op.n_qubits);
+        check_qubits(s, q);
+        if (q.size() != 1) throw Error(QFVM_ERR_VALUE, "noise channels act on one qubit");
+        const int64_t nk = op.body;
+        if (nk < 1 || nk > 64) throw Error(QFVM_ERR_VALUE, "channel operator count out of range");
+        const int64_t need_d = 2 * (4 * nk + (op.code == QFVM_OP_UCHAN ? nk : 0));
+        if (op.m_off < 0 || op.m_off + need_d > p->n_mat || !p->mat) throw Error(QFVM_ERR_VALUE, "matrix pool range");
+        auto mat = [&](int64_t k, cd m[4]) {
+            for (int e = 0; e < 4; e++)
+                m[e] = cd(p->mat[op.m_off + 8 * k + 2 * e], p->mat[op.m_off + 8 * k + 2 * e + 1]);
+        };
+        const double u = next_uniform();
+        flush();
+        cd m[4];
+        if (op.code == QFVM_OP_UCHAN) {  // numpy's choice: cdf / cdf[-1], first k with u < cdf[k]
+            std::vector<double> cdf((size_t)nk);
+            double acc = 0.0;
+            for (int64_t k = 0; k < nk; k++) {
+                acc += p->mat[op.m_off + 8 * nk + 2 * k];
+                cdf[(size_t)k] = acc;
+            }
+            int64_t pick = nk - 1;
+            for (int64_t k = 0; k < nk; k++)
+                if (u < cdf[(size_t)k] / cdf.back()) {
+                    pick = k;
+                    break;
+                }
+            mat(pick, m);
+            pending.push_back(one(G_DENSE, q[0], {}, m[0], m[1], m[2], m[3]));
+            return;
+        }
+        // Kraus: running sum of |K_k psi| / |psi| (simulator.py:59-74)
+        const double norm0 = std::sqrt(reduce_norm2(s));
+        double acc = 0.0;
+        for (int64_t k = 0; k < nk; k++) {
+            mat(k, m);
+            const double norm1 = std::sqrt(reduce_op_norm2(s, q[0], m));
+            acc += norm1 / norm0;
+            if (u < acc) {
+                const double inv = 1.0 / norm1;
+                pending.push_back(one(G_DENSE, q[0], {}, m[0] * inv, m[1] * inv, m[2] * inv, m[3] * inv));
+                return;
+            }
+        }
+    }
+
     void exec(int64_t begin, int64_t end) {
         int64_t k = begin;
         while (k < end) {
@@ -318,6 +366,9 @@ struct Runner {
                 std::vector<int> q = op_list(p, op.q_off, op.n_qubits);
                 check_qubits(s, q);
                 measure_like(q, true, nullptr);
+            } else if (op.code == QFVM_OP_KRAUS || op.code == QFVM_OP_UCHAN) {
+                k++;
+                channel(op);
             } else if (op.code == QFVM_OP_CIF) {
                 std::vector<int> c = op_list(p, op.c_off, op.n_cbits);
                 const int64_t body_end = k + 1 + op.body;

@@ -214,6 +214,8 @@ void sample_tree(State& s, const double* host_u, int64_t shots, uint64_t* host_o
 double apply_hamiltonian(State& s, const Hamiltonian& H, void* out, double* dev_result = nullptr);
 // grads_dev[g.slot] = Im <lam| P (controls projected) |psi> for one G_GRAD marker, over the whole state
 void launch_grad_marker(State& s, const Gate& g, const void* lam, double* grads_dev);
+// |K psi|^2 for a one-qubit operator K = [[m0, m1], [m2, m3]] on qubit q (one read of the state)
+double reduce_op_norm2(State& s, int q, const cd m[4]);
 // amplitudes at host-given indices (out: re, im pairs) and <a|b> of two states of one size
 void gather_amplitudes(State& s, const uint64_t* host_idx, uint64_t n, double* host_out);
 void reduce_inner(State& a, State& b, double* re, double* im);

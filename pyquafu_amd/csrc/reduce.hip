@@ -399,6 +399,44 @@ void launch_column_sums(State& s, const double* partials, uint64_t tiles, int co
     QFVM_LAUNCHED();
 }
 
+// |K psi|^2 for a one-qubit operator K on qubit q: sum over pairs of |k00 a + k01 b|^2 + |k10 a + k11 b|^2
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_op_norm2(const V2<R>* __restrict__ psi, uint64_t pairs, int q,
+                                                         double2 k00, double2 k01, double2 k10, double2 k11,
+                                                         double* partial) {
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t low = (1ull << q) - 1;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < pairs; j += stride) {
+        const uint64_t i0 = (j & low) | ((j & ~low) << 1);
+        const V2<R> a = psi[i0], b = psi[i0 | (1ull << q)];
+        const double ax = a.x, ay = a.y, bx = b.x, by = b.y;
+        const double u = k00.x * ax - k00.y * ay + k01.x * bx - k01.y * by;
+        const double v = k00.x * ay + k00.y * ax + k01.x * by + k01.y * bx;
+        const double w = k10.x * ax - k10.y * ay + k11.x * bx - k11.y * by;
+        const double z = k10.x * ay + k10.y * ax + k11.x * by + k11.y * bx;
+        acc += u * u + v * v + w * w + z * z;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+double reduce_op_norm2(State& s, int q, const cd m[4]) {
+    double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
+    const uint64_t pairs = s.size >> 1;
+    const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(pairs, RED_BLOCK));
+    const double2 k00 = make_double2(m[0].real(), m[0].imag()), k01 = make_double2(m[1].real(), m[1].imag());
+    const double2 k10 = make_double2(m[2].real(), m[2].imag()), k11 = make_double2(m[3].real(), m[3].imag());
+    if (s.dtype == QFVM_C128)
+        hipLaunchKernelGGL(k_op_norm2<double>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const double2*>(s.psi), pairs, q, k00, k01, k10, k11, partial);
+    else
+        hipLaunchKernelGGL(k_op_norm2<float>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
+                           reinterpret_cast<const float2*>(s.psi), pairs, q, k00, k01, k10, k11, partial);
+    QFVM_LAUNCHED();
+    return finish_sum(s, partial, grid);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Amplitudes at given indices, and the overlap <a|b> of two states of one size (any dtypes).
 template <typename R>

@@ -20,8 +20,8 @@ def simulate(
 ):
     """Simulate a circuit; returns a SimuResult.
 
-    Only the "statevector" simulator is provided (the reference's "clifford" and noisy simulators
-    are other engines). `precision` ("c128" default, or "c64") and `seed` are extensions."""
+    "statevector" and "noisy statevetor" (trajectories, the reference's spelling) run on the
+    engine; the reference's "clifford" simulator is another engine and is not provided. `precision` ("c128" default, or "c64") and `seed` are extensions."""
     if isinstance(qc, str):
         raise NotImplementedError("OpenQASM input is not supported by this engine; build a QuantumCircuit")
     if simulator == "statevector":
@@ -29,7 +29,12 @@ def simulate(
 
         backend = SVSimulator(use_gpu, use_custatevec, precision=precision, seed=seed)
         return backend.run(qc, psi, shots, hamiltonian)
-    if simulator in ("noisy statevetor", "clifford"):
+    if simulator == "noisy statevetor":  # the reference's spelling (simulators/__init__.py:66)
+        from .simulator import NoiseSVSimulator  # pylint: disable=import-outside-toplevel
+
+        backend = NoiseSVSimulator(use_gpu, use_custatevec, precision=precision, seed=seed)
+        return backend.run(qc, psi, shots, hamiltonian)
+    if simulator == "clifford":
         raise NotImplementedError(f"simulator {simulator!r} is not part of this engine")
     raise ValueError("invalid simulator name")
 

@@ -28,7 +28,8 @@ from typing import List, Tuple
 import numpy as np
 
 # op codes (include/qfvm.h)
-GENERAL, MULTI, X, Y, Z, H, S, SDG, T, TDG, P, RX, RY, RZ, CX, CZ, CP, CCX, RZZ, MEASURE, RESET, CIF = range(22)
+GENERAL, MULTI, X, Y, Z, H, S, SDG, T, TDG, P, RX, RY, RZ, CX, CZ, CP, CCX, RZZ, MEASURE, RESET, CIF, KRAUS, UCHAN = \
+    range(24)
 
 NAMED = {
     "x": X, "y": Y, "z": Z, "h": H, "s": S, "sdg": SDG, "t": T, "tdg": TDG, "p": P, "rx": RX, "ry": RY,
@@ -65,7 +66,7 @@ class Program:
     n_cbits: int = 0
     measure_vec: List[Tuple[int, int]] = field(default_factory=list)
     final_measure: bool = True
-    n_random: int = 0  # MEASURE + RESET ops anywhere in the stream (uniforms one shot may need)
+    n_random: int = 0  # MEASURE, RESET and channel ops in the stream (uniforms one shot may need)
 
     @property
     def n_ops(self) -> int:
@@ -73,7 +74,7 @@ class Program:
 
     @property
     def has_classical(self) -> bool:
-        return bool(np.isin(self.ops["code"], [MEASURE, RESET, CIF]).any()) if len(self.ops) else False
+        return bool(np.isin(self.ops["code"], [MEASURE, RESET, CIF, KRAUS, UCHAN]).any()) if len(self.ops) else False
 
 
 class ProgramBuilder:
@@ -134,6 +135,21 @@ def _emit_pyop(b: ProgramBuilder, op) -> str:
     if name == "reset":
         b.add(RESET, qubits=list(op.pos))
         return "reset"
+    if hasattr(op, "gatelist"):  # noise channel (elements/noise.py)
+        mats = [np.asarray(g.matrix, dtype=np.complex128) for g in op.gatelist]
+        if any(m.shape != (2, 2) for m in mats):
+            raise ValueError(f"channel {op.name}: only one-qubit operators are supported")
+        if hasattr(op, "probs"):
+            probs = np.asarray(op.probs, dtype=np.float64)
+            if probs.shape != (len(mats),):
+                raise ValueError(f"channel {op.name}: one probability per gate")
+            b.add(UCHAN, qubits=list(op.pos), matrix=np.concatenate([m.reshape(-1) for m in mats] +
+                                                                    [probs.astype(np.complex128)]),
+                  body=len(mats))
+        else:
+            b.add(KRAUS, qubits=list(op.pos), matrix=np.concatenate([m.reshape(-1) for m in mats]),
+                  body=len(mats))
+        return "channel"
     if name == "cif":
         row = b.add(CIF, cbits=list(op.cbits), cond=int(op.condition))
         start = len(b.rows)
@@ -189,7 +205,7 @@ def compile_circuit(qc) -> Program:
         measure_vec=[(int(q), int(c)) for q, c in state["measure_vec"]],
         final_measure=state["final_measure"],
     )
-    prog.n_random = int(np.isin(prog.ops["code"], [MEASURE, RESET]).sum()) if prog.n_ops else 0
+    prog.n_random = int(np.isin(prog.ops["code"], [MEASURE, RESET, KRAUS, UCHAN]).sum()) if prog.n_ops else 0
     return prog
 
 

@@ -162,5 +162,12 @@ def sampling_statevec(measures: dict, inputstate: np.ndarray, shots: int, *, see
     idx = _call(dev.sample, _uniforms(seed, int(shots)))
     meas = [(int(q), int(c)) for q, c in measures.items()]
     m = len(meas)
-    keys = counts_from_indices(idx, meas)
-    return {bin(k)[2:].zfill(m): v for k, v in keys.items()}
+    return bitstring_counts(idx, meas)
+
+
+def bitstring_counts(indices: np.ndarray, measure_vec: Sequence[Tuple[int, int]]) -> Dict[str, int]:
+    """counts_from_indices with the keys as measure_samples' strings (statevector.hpp:1381-1403):
+    len(measure_vec) characters, "" when nothing is measured."""
+    m = len(measure_vec)
+    keys = counts_from_indices(np.asarray(indices, dtype=np.uint64), measure_vec)
+    return {(bin(k)[2:].zfill(m) if m else ""): v for k, v in keys.items()}

@@ -9,10 +9,13 @@ from __future__ import annotations
 
 import numpy as np
 
+from .. import _native
+from ..elements.noise import KrausChannel, UnitaryChannel
 from ..elements.quantum_gate import CircuitWrapper, QuantumGate
 from ..exceptions import QuafuError
 from ..results.results import SimuResult
-from .qfvm import applyop_statevec, expect_statevec, simulate_circuit
+from .program import KRAUS, UCHAN, compile_gates
+from .qfvm import _call, applyop_statevec, bitstring_counts, expect_statevec, simulate_circuit
 
 
 class Simulator:
@@ -70,4 +73,89 @@ class SVSimulator(Simulator):
         res_info["qbitnum"] = qc.num
         res_info["measures"] = qc.measures
         res_info["simulator"] = "statevector"
+        return SimuResult(res_info)
+
+
+class NoiseSVSimulator(Simulator):
+    """Noisy statevector trajectories (quafu/simulators/simulator.py:148-223).
+
+    Each shot runs one trajectory of the circuit's gates and noise channels on the engine (channels
+    are program ops, include/qfvm.h QFVM_OP_KRAUS / QFVM_OP_UCHAN, one uniform each), then draws
+    one sample over `qc.measures` and, with a Hamiltonian, the coefficient-weighted Pauli
+    expectations, averaged over the shots. As in the reference's gen_circuit, only gates and
+    channels run: measure, reset and cif instructions are left out of the trajectory.
+    `seed` (an extension) seeds the trajectory uniforms."""
+
+    def __init__(self, use_gpu: bool = False, use_custatevec: bool = False, precision: str = "c128", seed=None):
+        self.backend = SVSimulator(use_gpu=use_gpu, use_custatevec=use_custatevec, precision=precision, seed=seed)
+        self.precision = precision
+        self.seed = seed
+
+    @staticmethod
+    def trajectory_program(qc):
+        """The program one trajectory runs: gates and channels in circuit order."""
+        if qc._has_wrap:  # pylint: disable=protected-access
+            qc.unwrap()
+        ops = [op for op in qc.instructions if isinstance(op, (QuantumGate, KrausChannel))]
+        prog, _ = compile_gates(ops, qc.num)
+        prog.n_random = int(np.isin(prog.ops["code"], [KRAUS, UCHAN]).sum()) if prog.n_ops else 0
+        return prog
+
+    @staticmethod
+    def gen_circuit(qc, rng=None):
+        """One sampled circuit: unitary channels replaced by a drawn gate (identities dropped),
+        Kraus channels kept between wrapped gate blocks (simulator.py:199-223)."""
+        from ..circuits import QuantumCircuit  # pylint: disable=import-outside-toplevel
+
+        num = qc.num
+        new_qc, temp_qc = QuantumCircuit(num), QuantumCircuit(num)
+        if qc._has_wrap:  # pylint: disable=protected-access
+            qc.unwrap()
+        for op in qc.instructions:
+            if isinstance(op, QuantumGate):
+                temp_qc << op  # pylint: disable=pointless-statement
+            elif isinstance(op, UnitaryChannel):
+                g = op.gen_gate(rng)
+                if g.name != "ID":
+                    temp_qc << g  # pylint: disable=pointless-statement
+            elif isinstance(op, KrausChannel):
+                new_qc << temp_qc.wrap()  # pylint: disable=pointless-statement
+                new_qc << op  # pylint: disable=pointless-statement
+                temp_qc = QuantumCircuit(num)
+        new_qc << temp_qc.wrap()  # pylint: disable=pointless-statement
+        return new_qc
+
+    def run(self, qc, psi: np.ndarray = np.array([]), shots: int = 0, hamiltonian=None):  # pylint: disable=arguments-differ
+        prog = self.trajectory_program(qc)
+        n = qc.num
+        has_input = psi is not None and np.asarray(psi).size != 0
+        if has_input and np.asarray(psi).size != (1 << n):
+            raise ValueError(f"input state length {np.asarray(psi).size} does not match {n} qubits")
+        shots = int(shots)
+        meas = [(int(q), int(c)) for q, c in qc.measures.items()]
+        paulis = hamiltonian.paulis if hamiltonian else []
+        terms = [(p.paulistr, [int(x) for x in p.pos]) for p in paulis]
+        coeffs = np.array([p.coeff for p in paulis])
+        if shots == 0:  # the reference divides its float accumulator by the shot count
+            raise ZeroDivisionError("float division by zero")
+        rng = np.random.default_rng(self.seed)
+        dev = _call(_native.pool.get, n, self.precision)
+        samples = np.zeros(max(shots, 0), dtype=np.uint64)
+        acc = np.zeros(len(paulis))
+        for shot in range(shots):
+            if has_input:
+                _call(dev.load, np.asarray(psi, dtype=np.complex128))
+            else:
+                _call(dev.init_zero)
+            u = rng.random(prog.n_random + 1)
+            _call(dev.run, prog, u[:-1], 0, True)
+            samples[shot] = _call(dev.sample, u[-1:])[0]
+            if paulis:
+                acc += _call(dev.expect, terms)
+        counts = bitstring_counts(samples, meas) if shots > 0 else {}
+        pauli_expects = list(acc * coeffs / shots) if paulis else []
+        res_info = {"counts": counts, "pauli_expects": pauli_expects}
+        res_info["qbitnum"] = qc.num
+        res_info["measures"] = qc.measures
+        res_info["simulator"] = "noisy statevector"
         return SimuResult(res_info)
