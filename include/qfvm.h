@@ -237,6 +237,14 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *   "low_bits"    low qubits always in a tile L (0 = default 6)
  *   "max_groups"  4-qubit register groups per fused pass (1..24, default 12)
  *   "pass_cost"   compute budget of one fused pass in quarter dense-gate units (0 = default)
+ *   "split_rows"  1 = (default) complex128 one-state passes of 2^11 or 2^12-amplitude tiles run
+ *                 with two lanes per 16-amplitude register row (twice the wavefronts per tile),
+ *                 0 = one lane per row
+ *   "stream"      1 = persistent fused-pass kernel that prefetches the next tile (complex128,
+ *                 one-state passes of 2^11..2^13-amplitude tiles, when split_rows does not apply),
+ *                 0 = one workgroup per tile
+ *   "stagger"     persistent kernel: start delay of half the workgroups, in 8128-cycle sleeps (0..64)
+ *   "stagger_shift" the blockIdx bit that selects the delayed half (0..16)
  * Results do not depend on the options beyond floating-point rounding.
  */
 int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);

@@ -799,6 +799,16 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
         } else if (k == "pass_cost") {
             if (value < 0 || value > (1 << 20)) throw Error(QFVM_ERR_VALUE, "pass_cost must be in 0..2^20");
             s->pass_cost = (int)value;
+        } else if (k == "split_rows") {
+            s->split_rows = value != 0;
+        } else if (k == "stream") {
+            s->stream_tiles = value != 0;
+        } else if (k == "stagger") {
+            if (value < 0 || value > 64) throw Error(QFVM_ERR_VALUE, "stagger must be in 0..64");
+            s->stagger = (int)value;
+        } else if (k == "stagger_shift") {
+            if (value < 0 || value > 16) throw Error(QFVM_ERR_VALUE, "stagger_shift must be in 0..16");
+            s->stagger_shift = (int)value;
         } else {
             throw Error(QFVM_ERR_VALUE, "unknown option '" + k + "'");
         }

@@ -141,6 +141,10 @@ struct State {
     int low_bits = 0;
     int max_groups = 12;
     int pass_cost = 0;  // fused-pass compute budget (quarter dense-gate units; 0 = unlimited)
+    int stream_tiles = 0;   // persistent fused-pass kernel with next-tile prefetch (complex128)
+    int split_rows = 1;     // fused-pass kernel with two lanes per 16-amplitude row (complex128)
+    int stagger = 0;        // start delay of half the persistent workgroups, in 8128-cycle sleeps
+    int stagger_shift = 0;  // which blockIdx bit picks the delayed half
     // counters: always on; event timing only when `profiling`
     bool profiling = false;
     std::vector<std::pair<std::string, KStat>> kstats;

@@ -453,6 +453,412 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
 }
 
 // ------------------------------------------------------------------------------------------------
+// Persistent variant for one-state complex128 passes with one row per thread (2^11 .. 2^13
+// amplitudes per tile). Each workgroup walks the tiles blockIdx.x, blockIdx.x + gridDim.x, ...; as
+// soon as every thread has gathered its row of the last group, the next tile's HBM -> LDS load is
+// issued, so it runs under the last group's arithmetic, and the last group stores its registers
+// straight to HBM (register member P holds logical member P ^ px: global index G0 | bits(P ^ px)).
+// `stagger` delays the workgroups with bit `stagger_shift` of blockIdx set by that many 8128-cycle
+// sleeps before their first tile, so the load and compute phases of co-resident workgroups can be
+// set apart. Algorithmic bytes per launch: 2 * 16 * 2^n, as k_fused_pass.
+template <int... Ms>
+__device__ __forceinline__ void member_offsets(uint64_t (&gm)[16], uint64_t q0, uint64_t q1, uint64_t q2, uint64_t q3,
+                                               std::integer_sequence<int, Ms...>) {
+    ((gm[Ms] = ((Ms & 1) ? q0 : 0) | ((Ms & 2) ? q1 : 0) | ((Ms & 4) ? q2 : 0) | ((Ms & 8) ? q3 : 0)), ...);
+}
+
+template <typename R, int... Ms>
+__device__ __forceinline__ void store_seq(const V2<R> (&v)[16], V2<R>* __restrict__ psi, uint64_t g,
+                                          const uint64_t (&gm)[16], std::integer_sequence<int, Ms...>) {
+    ((psi[g ^ gm[Ms]] = v[Ms]), ...);
+}
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const PassDesc& pd) {
+    for (int k = 0; k < pd.b; k++) {
+        const uint64_t low = (1ull << pd.q[k]) - 1;
+        t = (t & low) | ((t & ~low) << 1);
+    }
+    return t;
+}
+
+__global__ void __launch_bounds__(512) k_fused_stream(double2* __restrict__ psi, const PassDesc pd,
+                                                       const FOpDev<double>* __restrict__ ops,
+                                                       const unsigned char* __restrict__ runs, uint64_t tiles,
+                                                       uint32_t stagger, uint32_t stagger_shift) {
+    extern __shared__ unsigned char lds_raw[];
+    double2* tile = reinterpret_cast<double2*>(lds_raw);
+    const int T = 1 << pd.b;
+    const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 16: one row per thread
+    const int wave = tid >> 6;
+    uint64_t t = blockIdx.x;
+    if (t >= tiles) return;
+    if (stagger != 0 && ((blockIdx.x >> stagger_shift) & 1u))
+        for (uint32_t i = 0; i < stagger; i++) __builtin_amdgcn_s_sleep(127);
+    const uint64_t l_sw = local_to_global(swz(tid), pd);
+    auto issue_load = [&](uint64_t base) {
+        // LDS slot lh + tid receives element lh + swz(tid) (see k_fused_pass)
+#pragma unroll 16
+        for (int lh = 0; lh < T; lh += NT) {
+            const void* src = (const void*)(psi + (base | l_sw | local_to_global(lh, pd)));
+            auto* dst = (__attribute__((address_space(3))) void*)(tile + lh + wave * 64);
+            __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+        }
+    };
+    uint64_t base = tile_base(t, pd);
+    issue_load(base);
+    const int last = pd.ngroups - 1;
+    // the last group's member offsets in global index space (uniform)
+    const int ls0 = pd.gbits[last][0], ls1 = pd.gbits[last][1], ls2 = pd.gbits[last][2], ls3 = pd.gbits[last][3];
+    uint64_t gm[16];
+    member_offsets(gm, 1ull << pd.q[ls0], 1ull << pd.q[ls1], 1ull << pd.q[ls2], 1ull << pd.q[ls3],
+                   std::make_integer_sequence<int, 16>{});
+    const int l_last = ins_bit(ins_bit(ins_bit(ins_bit(tid, ls0), ls1), ls2), ls3);
+    int a_last[16];
+    row_slots(a_last, l_last, ls0, ls1, ls2, ls3, std::make_integer_sequence<int, 16>{});
+    const uint64_t g_last = local_to_global(l_last, pd);
+    for (;;) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int gi = 0; gi < last; gi++) {
+            const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];
+            const int l0 = ins_bit(ins_bit(ins_bit(ins_bit(tid, s0), s1), s2), s3);
+            int a[16];
+            row_slots(a, l0, s0, s1, s2, s3, std::make_integer_sequence<int, 16>{});
+            const uint64_t G0 = base | local_to_global(l0, pd);
+            double2 v[16];
+            gather_seq<double>(v, tile, a, std::make_integer_sequence<int, 16>{});
+            for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_op<double>(v, ops[oi], G0, runs);
+            const uint32_t pc = pd.gpxc[gi];
+            const int xo = swz((px_bit(G0, pd.gpxm[gi][0], pc) << s0) | (px_bit(G0, pd.gpxm[gi][1], pc >> 1) << s1) |
+                               (px_bit(G0, pd.gpxm[gi][2], pc >> 2) << s2) |
+                               (px_bit(G0, pd.gpxm[gi][3], pc >> 3) << s3));
+            scatter_seq<double>(v, tile, a, xo, std::make_integer_sequence<int, 16>{});
+            __syncthreads();
+        }
+        const uint64_t G0 = base | g_last;
+        double2 v[16];
+        gather_seq<double>(v, tile, a_last, std::make_integer_sequence<int, 16>{});
+        __syncthreads();  // every row of the tile is in registers: LDS is free for the next tile
+        const uint64_t tn = t + gridDim.x;
+        const uint64_t base_n = tn < tiles ? tile_base(tn, pd) : 0;
+        if (tn < tiles) issue_load(base_n);
+        for (int oi = pd.gop[last]; oi < pd.gop[last + 1]; oi++) apply_op<double>(v, ops[oi], G0, runs);
+        const uint32_t pc = pd.gpxc[last];
+        const uint64_t pxg = (px_bit(G0, pd.gpxm[last][0], pc) ? gm[1] : 0) |
+                             (px_bit(G0, pd.gpxm[last][1], pc >> 1) ? gm[2] : 0) |
+                             (px_bit(G0, pd.gpxm[last][2], pc >> 2) ? gm[4] : 0) |
+                             (px_bit(G0, pd.gpxm[last][3], pc >> 3) ? gm[8] : 0);
+        store_seq<double>(v, psi, G0 ^ pxg, gm, std::make_integer_sequence<int, 16>{});
+        if (tn >= tiles) break;
+        t = tn;
+        base = base_n;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Split-row variant (one-state complex128 passes of 2^11 or 2^12 amplitudes): two lanes (lane,
+// lane ^ 1) share each 16-member row; lane bit 0 = h holds the members with group bit 3 equal to h,
+// register m <-> member (h << 3) | m. A tile then has 2^(b-3) threads, twice the waves of
+// k_fused_pass, to stream it in and out, and a thread holds 8 amplitudes instead of 16. Gates on
+// group bits 0..2 stay lane-local; a gate on group bit 3 pairs register m of the two lanes: the
+// partner's value comes over a DPP quad permutation and each lane computes its own output with
+// per-lane coefficients (m00, m01 where h = 0; m11, m10 where h = 1). Controls on group bit 3 and
+// the diagonal-run subsets that contain it become per-lane conditions.
+
+// the partner lane's (lane ^ 1) copy of a value; every lane of the wavefront must be active
+__device__ __forceinline__ double xlane(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0xB1, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double2 xlane(double2 v) { return make_double2(xlane(v.x), xlane(v.y)); }
+
+// 2x2 kinds on a lane-local group bit G < 3 (pairs m, m | 1 << G) with group controls cm7 on bits 0..2
+// (CTRL = false: no control among bits 0..2, no per-pair tests)
+template <int KIND, int G, int M, bool CTRL>
+__device__ __forceinline__ void hpair_local(double2 (&v)[8], int cm7, double2 m00, double2 m01, double2 m10,
+                                            double2 m11) {
+    if constexpr ((M >> G) & 1) {
+        return;
+    } else {
+        if constexpr (CTRL) {
+            if ((M & cm7) != cm7) return;
+        }
+        constexpr int M1 = M | (1 << G);
+        const double2 a = v[M], b = v[M1];
+        if constexpr (KIND == F_SWAP) {
+            v[M] = b;
+            v[M1] = a;
+        } else if constexpr (KIND == F_DIAG) {
+            v[M] = cmul<double>(m00, a);
+            v[M1] = cmul<double>(m11, b);
+        } else if constexpr (KIND == F_REAL) {
+            v[M] = mk<double>(m00.x * a.x + m01.x * b.x, m00.x * a.y + m01.x * b.y);
+            v[M1] = mk<double>(m10.x * a.x + m11.x * b.x, m10.x * a.y + m11.x * b.y);
+        } else {
+            v[M] = cmul2<double>(m00, a, m01, b);
+            v[M1] = cmul2<double>(m10, a, m11, b);
+        }
+    }
+}
+
+// 2x2 kinds on group bit 3 (across the lane pair): own coefficient c0, partner coefficient c1
+template <int KIND, int M, bool CTRL>
+__device__ __forceinline__ void hpair_cross(double2 (&v)[8], int cm7, double2 c0, double2 c1) {
+    if constexpr (CTRL) {
+        if ((M & cm7) != cm7) return;
+    }
+    if constexpr (KIND == F_DIAG) {
+        v[M] = cmul<double>(c0, v[M]);
+    } else if constexpr (KIND == F_SWAP) {
+        v[M] = xlane(v[M]);
+    } else {
+        const double2 o = xlane(v[M]);
+        if constexpr (KIND == F_REAL)
+            v[M] = mk<double>(c0.x * v[M].x + c1.x * o.x, c0.x * v[M].y + c1.x * o.y);
+        else
+            v[M] = cmul2<double>(c0, v[M], c1, o);
+    }
+}
+
+template <int KIND, int G, int... Ms>
+__device__ __forceinline__ void hpairs(double2 (&v)[8], int cm7, double2 m00, double2 m01, double2 m10, double2 m11,
+                                       std::integer_sequence<int, Ms...>) {
+    if constexpr (G < 3) {
+        if (cm7 == 0)
+            (hpair_local<KIND, G, Ms, false>(v, cm7, m00, m01, m10, m11), ...);
+        else
+            (hpair_local<KIND, G, Ms, true>(v, cm7, m00, m01, m10, m11), ...);
+    } else {  // m00 / m01 carry this lane's own / partner coefficients
+        if (cm7 == 0)
+            (hpair_cross<KIND, Ms, false>(v, cm7, m00, m01), ...);
+        else
+            (hpair_cross<KIND, Ms, true>(v, cm7, m00, m01), ...);
+    }
+}
+
+template <int KIND, int G>
+__device__ __forceinline__ void hop_2x2(double2 (&v)[8], const FOpDev<double>& o, bool ok, uint64_t G0, int h) {
+    double2 m00 = mk<double>(o.m[0], o.m[1]), m01 = mk<double>(o.m[2], o.m[3]);
+    double2 m10 = mk<double>(o.m[4], o.m[5]), m11 = mk<double>(o.m[6], o.m[7]);
+    if (o.flags & FL_TARGET_PX) {  // registers hold the pair swapped where the relabel bit is set
+        const bool sw = px_bit(G0, o.pxmask, (o.flags >> 1) & 1u);
+        const double2 t00 = m00, t01 = m01, t10 = m10, t11 = m11;
+        m00 = csel<double>(sw, t11, t00);
+        m01 = csel<double>(sw, t10, t01);
+        m10 = csel<double>(sw, t01, t10);
+        m11 = csel<double>(sw, t00, t11);
+    }
+    // lanes whose outside controls fail, or (G < 3) whose half misses a control on group bit 3,
+    // apply the identity
+    const bool lane_ok = ok && (G == 3 || !(o.cm_group & 8u) || h);
+    if (o.cm_ext != 0 || (G < 3 && (o.cm_group & 8u))) {
+        const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
+        m00 = csel<double>(lane_ok, m00, one);
+        m01 = csel<double>(lane_ok, m01, zero);
+        m10 = csel<double>(lane_ok, m10, zero);
+        m11 = csel<double>(lane_ok, m11, one);
+    }
+    if constexpr (G == 3) {
+        const double2 c0 = csel<double>(h, m11, m00), c1 = csel<double>(h, m10, m01);
+        hpairs<KIND, 3>(v, (int)(o.cm_group & 7u), c0, c1, c0, c1, std::make_integer_sequence<int, 8>{});
+    } else {
+        hpairs<KIND, G>(v, (int)(o.cm_group & 7u), m00, m01, m10, m11, std::make_integer_sequence<int, 8>{});
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ void hop_kind(double2 (&v)[8], const FOpDev<double>& o, bool ok, uint64_t G0, int h) {
+    switch (o.g) {
+        case 0: hop_2x2<KIND, 0>(v, o, ok, G0, h); break;
+        case 1: hop_2x2<KIND, 1>(v, o, ok, G0, h); break;
+        case 2: hop_2x2<KIND, 2>(v, o, ok, G0, h); break;
+        default: hop_2x2<KIND, 3>(v, o, ok, G0, h); break;
+    }
+}
+
+// X on group bit 3 with group controls cm7 (bits 0..2): registers take the partner's value where
+// the lane condition holds
+template <int... Ms>
+__device__ __forceinline__ void hswap_cross(double2 (&v)[8], int cm7, bool ok, std::integer_sequence<int, Ms...>) {
+    if (cm7 == 0)
+        ((v[Ms] = csel<double>(ok, xlane(v[Ms]), v[Ms])), ...);
+    else
+        ((((Ms & cm7) == cm7) ? (void)(v[Ms] = csel<double>(ok, xlane(v[Ms]), v[Ms])) : (void)0), ...);
+}
+
+// relabel flush of group bit G: pairs swap where sw is set (bitwise blend)
+template <int G, int M>
+__device__ __forceinline__ void hflush_step(double2 (&v)[8], uint64_t msk) {
+    if constexpr (G == 3) {
+        const double2 o = xlane(v[M]);
+        const uint64_t ax = __builtin_bit_cast(uint64_t, v[M].x), ay = __builtin_bit_cast(uint64_t, v[M].y);
+        const uint64_t bx = __builtin_bit_cast(uint64_t, o.x), by = __builtin_bit_cast(uint64_t, o.y);
+        v[M].x = __builtin_bit_cast(double, ax ^ ((ax ^ bx) & msk));
+        v[M].y = __builtin_bit_cast(double, ay ^ ((ay ^ by) & msk));
+    } else if constexpr (!((M >> G) & 1)) {
+        constexpr int M1 = M | (1 << G);
+        const uint64_t ax = __builtin_bit_cast(uint64_t, v[M].x), ay = __builtin_bit_cast(uint64_t, v[M].y);
+        const uint64_t bx = __builtin_bit_cast(uint64_t, v[M1].x), by = __builtin_bit_cast(uint64_t, v[M1].y);
+        const uint64_t tx = (ax ^ bx) & msk, ty = (ay ^ by) & msk;
+        v[M].x = __builtin_bit_cast(double, ax ^ tx);
+        v[M].y = __builtin_bit_cast(double, ay ^ ty);
+        v[M1].x = __builtin_bit_cast(double, bx ^ tx);
+        v[M1].y = __builtin_bit_cast(double, by ^ ty);
+    }
+}
+
+template <int G, int... Ms>
+__device__ __forceinline__ void hflush(double2 (&v)[8], int sw, std::integer_sequence<int, Ms...>) {
+    const uint64_t msk = 0ull - (uint64_t)sw;
+    (hflush_step<G, Ms>(v, msk), ...);
+}
+
+template <int... Ms>
+__device__ __forceinline__ void hphase(double2 (&v)[8], int cm7, double2 d, std::integer_sequence<int, Ms...>) {
+    if (cm7 == 0)
+        ((v[Ms] = cmul<double>(d, v[Ms])), ...);
+    else
+        ((((Ms & cm7) == cm7) ? (void)(v[Ms] = cmul<double>(d, v[Ms])) : (void)0), ...);
+}
+
+// factor f_S of a diagonal run applied to this lane's members (h << 3) | m that contain S
+template <int S, int... Ms>
+__device__ __forceinline__ void hrun_subset(double2 (&v)[8], const RunHead<double>* __restrict__ rh,
+                                            const RunTerm<double>* __restrict__ terms, uint32_t active,
+                                            uint32_t angles, uint64_t G0, int h, std::integer_sequence<int, Ms...>) {
+    if (!((active >> S) & 1u)) return;
+    double2 f = mk<double>(rh->f0[2 * S], rh->f0[2 * S + 1]);
+    double th = 0;
+    const int t1 = rh->tbeg[S + 1];
+    for (int t = rh->tbeg[S]; t < t1; t++) {
+        const RunTerm<double>& tm = terms[t];
+        const bool cond = (G0 & tm.cl) == tm.cl;
+        const bool sel = px_bit(G0, tm.pm, tm.pc);
+        if (tm.angle) {
+            th += cond ? (sel ? tm.v[1] : tm.v[0]) : 0.0;
+        } else {
+            double2 val = csel<double>(sel, mk<double>(tm.v[2], tm.v[3]), mk<double>(tm.v[0], tm.v[1]));
+            val = csel<double>(cond, val, mk<double>(1, 0));
+            f = cmul<double>(f, val);
+        }
+    }
+    if ((angles >> S) & 1u) {
+        double sn, cs;
+        sin_cos(th, &sn, &cs);
+        f = cmul<double>(f, mk<double>(cs, sn));
+    }
+    if constexpr ((S & 8) != 0) f = csel<double>(h, f, mk<double>(1, 0));
+    constexpr int S7 = S & 7;
+    ((((Ms & S7) == S7) ? (void)(v[Ms] = cmul<double>(f, v[Ms])) : (void)0), ...);
+}
+
+template <int... Ss>
+__device__ __forceinline__ void hdiag_run(double2 (&v)[8], const unsigned char* __restrict__ runs,
+                                          const FOpDev<double>& o, uint64_t G0, int h, std::integer_sequence<int, Ss...>) {
+    const RunHead<double>* rh = reinterpret_cast<const RunHead<double>*>(runs + o.cm_ext);
+    const RunTerm<double>* terms = reinterpret_cast<const RunTerm<double>*>(rh + 1);
+    (hrun_subset<Ss>(v, rh, terms, (uint32_t)o.g, o.flags, G0, h, std::make_integer_sequence<int, 8>{}), ...);
+}
+
+__device__ __forceinline__ void apply_hop(double2 (&v)[8], const FOpDev<double>& o, uint64_t G0, int h,
+                                          const unsigned char* __restrict__ runs) {
+    const bool ok = (G0 & o.cm_ext) == o.cm_ext;
+    switch (o.kind) {
+        case F_DENSE: hop_kind<F_DENSE>(v, o, ok, G0, h); break;
+        case F_REAL: hop_kind<F_REAL>(v, o, ok, G0, h); break;
+        case F_DIAG: hop_kind<F_DIAG>(v, o, ok, G0, h); break;
+        case F_SWAP:
+            if (o.g == 3) {
+                hswap_cross(v, (int)(o.cm_group & 7u), ok, std::make_integer_sequence<int, 8>{});
+            } else if (o.cm_ext == 0 && !(o.cm_group & 8u)) {
+                hop_kind<F_SWAP>(v, o, ok, G0, h);  // uniform register moves
+            } else {
+                hop_kind<F_DENSE>(v, o, ok, G0, h);  // X as a matrix, identity where a condition fails
+            }
+            break;
+        case F_FLUSH: {
+            const int sw = px_bit(G0, o.pxmask, (o.flags >> 1) & 1u);
+            switch (o.g) {
+                case 0: hflush<0>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                case 1: hflush<1>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                case 2: hflush<2>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                default: hflush<3>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+            }
+            break;
+        }
+        case F_DIAGRUN: hdiag_run(v, runs, o, G0, h, std::make_integer_sequence<int, 16>{}); break;
+        default: {  // F_PHASE
+            double2 d = csel<double>((G0 >> o.g) & 1ull, mk<double>(o.m[6], o.m[7]), mk<double>(o.m[0], o.m[1]));
+            d = csel<double>(ok && (!(o.cm_group & 8u) || h), d, mk<double>(1, 0));
+            hphase(v, (int)(o.cm_group & 7u), d, std::make_integer_sequence<int, 8>{});
+        }
+    }
+}
+
+template <int... Ms>
+__device__ __forceinline__ void hrow_slots(int (&a)[8], int l0, int s0, int s1, int s2,
+                                           std::integer_sequence<int, Ms...>) {
+    ((a[Ms] = swz(l0 | ((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2))), ...);
+}
+
+template <int... Ms>
+__device__ __forceinline__ void hgather(double2 (&v)[8], const double2* tile, const int (&a)[8],
+                                        std::integer_sequence<int, Ms...>) {
+    ((v[Ms] = tile[a[Ms]]), ...);
+}
+
+template <int... Ms>
+__device__ __forceinline__ void hscatter(const double2 (&v)[8], double2* tile, const int (&a)[8], int xo,
+                                         std::integer_sequence<int, Ms...>) {
+    ((tile[a[Ms] ^ xo] = v[Ms]), ...);
+}
+
+__global__ void __launch_bounds__(512) k_fused_half(double2* __restrict__ psi, const PassDesc pd,
+                                                     const FOpDev<double>* __restrict__ ops,
+                                                     const unsigned char* __restrict__ runs) {
+    extern __shared__ unsigned char lds_raw[];
+    double2* tile = reinterpret_cast<double2*>(lds_raw);
+    const int T = 1 << pd.b;
+    const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 8: two lanes per row
+    const uint64_t base = tile_base(blockIdx.x, pd);
+    {
+        // direct HBM -> LDS loads: LDS slot lh + tid receives element lh + swz(tid)
+        const uint64_t g_sw = base | local_to_global(swz(tid), pd);
+        const int wave = tid >> 6;
+#pragma unroll 8
+        for (int lh = 0; lh < T; lh += NT) {
+            const void* src = (const void*)(psi + (g_sw | local_to_global(lh, pd)));
+            auto* dst = (__attribute__((address_space(3))) void*)(tile + lh + wave * 64);
+            __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int h = tid & 1, r = tid >> 1;
+    for (int gi = 0; gi < pd.ngroups; gi++) {
+        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];
+        const int l0 = ins_bit(ins_bit(ins_bit(ins_bit(r, s0), s1), s2), s3);
+        int a[8];
+        hrow_slots(a, l0 | (h << s3), s0, s1, s2, std::make_integer_sequence<int, 8>{});
+        const uint64_t G0 = base | local_to_global(l0, pd);
+        double2 v[8];
+        hgather(v, tile, a, std::make_integer_sequence<int, 8>{});
+        for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_hop(v, ops[oi], G0, h, runs);
+        const uint32_t pc = pd.gpxc[gi];
+        const int xo = swz((px_bit(G0, pd.gpxm[gi][0], pc) << s0) | (px_bit(G0, pd.gpxm[gi][1], pc >> 1) << s1) |
+                           (px_bit(G0, pd.gpxm[gi][2], pc >> 2) << s2) | (px_bit(G0, pd.gpxm[gi][3], pc >> 3) << s3));
+        // a row's two lanes are in one wavefront, whose LDS reads all precede its writes
+        hscatter(v, tile, a, xo, std::make_integer_sequence<int, 8>{});
+        __syncthreads();
+    }
+    const uint64_t g_tid = base | local_to_global(tid, pd);
+#pragma unroll 8
+    for (int lh = 0; lh < T; lh += NT) psi[g_tid | local_to_global(lh, pd)] = tile[swz(lh + tid)];
+}
+
+// ------------------------------------------------------------------------------------------------
 // Appends the record of the diagonal run `run` of group G to `runs` (16-byte aligned) and returns
 // the device op that applies it.
 template <typename R>
@@ -611,6 +1017,26 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
     return pd;
 }
 
+// Workgroups of a persistent k_fused_stream launch: as many as fit on the device at once.
+static unsigned stream_grid(const State& s, size_t lds, unsigned threads, uint64_t tiles) {
+    static std::vector<std::pair<uint64_t, unsigned>> known;  // (device, lds, threads) -> grid
+    const uint64_t key = ((uint64_t)s.device << 40) | ((uint64_t)lds << 12) | threads;
+    unsigned grid = 0;
+    for (const auto& kv : known)
+        if (kv.first == key) grid = kv.second;
+    if (grid == 0) {
+        QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_stream),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        int per_cu = 0, cus = 0;
+        QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused_stream),
+                                                              (int)threads, lds));
+        QFVM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device));
+        grid = (unsigned)std::max(1, per_cu * cus);
+        known.emplace_back(key, grid);
+    }
+    return (unsigned)std::min<uint64_t>(grid, tiles);
+}
+
 template <typename R, int NS>
 static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam, double* grads_dev) {
     // encode every pass's ops (and marker slots) into one array, upload once, then launch the
@@ -681,6 +1107,26 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         }
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
+        if constexpr (NS == 1 && sizeof(R) == 8) {
+            if (s.split_rows && (pd.b == 11 || pd.b == 12)) {
+                if (lds > 65536) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 64 KiB");
+                KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
+                hipLaunchKernelGGL(k_fused_half, dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds, s.stream,
+                                   reinterpret_cast<double2*>(psi), pd, reinterpret_cast<const FOpDev<double>*>(ops),
+                                   druns);
+                QFVM_LAUNCHED();
+                continue;
+            }
+            if (s.stream_tiles && pd.b >= 11 && pd.b <= 13 && pd.ngroups >= 1 && pd.ngroups <= FUSE_MAX_GROUPS) {
+                const unsigned grid = stream_grid(s, lds, threads, tiles);
+                KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
+                hipLaunchKernelGGL(k_fused_stream, dim3(grid), dim3(threads), lds, s.stream,
+                                   reinterpret_cast<double2*>(psi), pd, reinterpret_cast<const FOpDev<double>*>(ops),
+                                   druns, tiles, (uint32_t)s.stagger, (uint32_t)s.stagger_shift);
+                QFVM_LAUNCHED();
+                continue;
+            }
+        }
         // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
         // between passes stages its matrix there)
         double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;

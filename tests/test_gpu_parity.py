@@ -289,9 +289,17 @@ def test_errors_are_reported():
 
 
 # ------------------------------------------------------------------ fused passes
-@pytest.mark.parametrize("dtype,tile,low,groups", [("c128", 12, 6, 12), ("c128", 12, 3, 4), ("c128", 13, 4, 24),
-                                                   ("c128", 8, 2, 2), ("c64", 13, 6, 12), ("c64", 14, 4, 24)])
-def test_fused_matches_oracle(dtype, tile, low, groups):
+# kernel variants of complex128 one-state passes: split rows (the default for 2^11 / 2^12 tiles),
+# one lane per row, and the persistent prefetching kernel
+KERNELS = {"split": (("split_rows", 1),), "classic": (("split_rows", 0),), "stream": (("split_rows", 0), ("stream", 1))}
+
+
+@pytest.mark.parametrize("dtype,tile,low,groups,kernel", [
+    ("c128", 12, 6, 12, "split"), ("c128", 12, 6, 12, "classic"), ("c128", 12, 6, 12, "stream"),
+    ("c128", 11, 5, 12, "split"), ("c128", 11, 3, 6, "stream"), ("c128", 12, 3, 4, "split"),
+    ("c128", 12, 3, 4, "classic"), ("c128", 13, 4, 24, "stream"), ("c128", 13, 4, 24, "classic"),
+    ("c128", 8, 2, 2, "split"), ("c64", 13, 6, 12, "split"), ("c64", 14, 4, 24, "split")])
+def test_fused_matches_oracle(dtype, tile, low, groups, kernel):
     rng = np.random.Generator(np.random.PCG64(tile * 100 + low))
     n = 17
     prog = random_gate_program(n, 400, rng, max_multi=3)
@@ -300,7 +308,7 @@ def test_fused_matches_oracle(dtype, tile, low, groups):
     cpu.load(psi0)
     cpu.apply(prog)
     dev = gpu_state(n, dtype)
-    for k, v in (("tile_bits", tile), ("low_bits", low), ("max_groups", groups)):
+    for k, v in (("tile_bits", tile), ("low_bits", low), ("max_groups", groups)) + KERNELS[kernel]:
         dev.set_option(k, v)
     dev.load(psi0)
     dev.apply(prog)
@@ -308,11 +316,14 @@ def test_fused_matches_oracle(dtype, tile, low, groups):
     assert dev.profile()["fused_pass"]["launches"] > 0
 
 
-def test_fused_equals_unfused_at_24_qubits():
+@pytest.mark.parametrize("kernel", ["split", "classic", "stream"])
+def test_fused_equals_unfused_at_24_qubits(kernel):
     """The two GPU paths on the HBM workload's structure at 24 qubits, and the oracle."""
     n = 24
     prog = compile_circuit(workloads.random_u3_cx(n, 6, 15, seed=31, measure=False))
     a = gpu_state(n)
+    for k, v in KERNELS[kernel]:
+        a.set_option(k, v)
     a.apply(prog)
     fa = a.store()
     a.close()
@@ -401,8 +412,9 @@ def diagonal_heavy_program(n: int, n_ops: int, rng):
     return b.program(n_qubits=n, n_cbits=n)
 
 
-@pytest.mark.parametrize("n,dtype", [(9, "c128"), (13, "c128"), (16, "c128"), (13, "c64")])
-def test_diagonal_runs_match_oracle(n, dtype):
+@pytest.mark.parametrize("n,dtype,kernel", [(9, "c128", "split"), (13, "c128", "split"), (16, "c128", "split"),
+                                            (13, "c128", "classic"), (16, "c128", "stream"), (13, "c64", "split")])
+def test_diagonal_runs_match_oracle(n, dtype, kernel):
     rng = np.random.Generator(np.random.PCG64(1900 + n))
     prog = diagonal_heavy_program(n, 400, rng)
     psi0 = rand_state(n, rng)
@@ -410,6 +422,8 @@ def test_diagonal_runs_match_oracle(n, dtype):
     cpu.load(psi0)
     cpu.apply(prog)
     dev = _native.DeviceState(n, dtype)
+    for k, v in KERNELS[kernel]:
+        dev.set_option(k, v)
     dev.load(psi0)
     dev.apply(prog)
     got = dev.store()
