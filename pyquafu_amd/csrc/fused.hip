@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(512) k_fused_stream(double2* __restrict__ psi,
 }
 
 // ------------------------------------------------------------------------------------------------
-// Split-row variant (one-state complex128 passes of 2^11 or 2^12 amplitudes): two lanes (lane,
+// Split-row variant (one-state complex128 passes of 2^11 .. 2^13 amplitudes): two lanes (lane,
 // lane ^ 1) share each 16-member row; lane bit 0 = h holds the members with group bit 3 equal to h,
 // register m <-> member (h << 3) | m. A tile then has 2^(b-3) threads, twice the waves of
 // k_fused_pass, to stream it in and out, and a thread holds 8 amplitudes instead of 16. Gates on
@@ -797,10 +797,13 @@ __device__ __forceinline__ void apply_hop(double2 (&v)[8], const FOpDev<double>&
     }
 }
 
+// LDS slots of a lane's 8 members: the swizzle is linear over XOR and the member bits are disjoint
+// from the row's bits, so slot(m) = swz(row) ^ swz(member offset), the second term uniform
 template <int... Ms>
 __device__ __forceinline__ void hrow_slots(int (&a)[8], int l0, int s0, int s1, int s2,
                                            std::integer_sequence<int, Ms...>) {
-    ((a[Ms] = swz(l0 | ((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2))), ...);
+    const int sl = swz(l0);
+    ((a[Ms] = sl ^ swz(((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2))), ...);
 }
 
 template <int... Ms>
@@ -815,7 +818,7 @@ __device__ __forceinline__ void hscatter(const double2 (&v)[8], double2* tile, c
     ((tile[a[Ms] ^ xo] = v[Ms]), ...);
 }
 
-__global__ void __launch_bounds__(512) k_fused_half(double2* __restrict__ psi, const PassDesc pd,
+__global__ void __launch_bounds__(1024) k_fused_half(double2* __restrict__ psi, const PassDesc pd,
                                                      const FOpDev<double>* __restrict__ ops,
                                                      const unsigned char* __restrict__ runs) {
     extern __shared__ unsigned char lds_raw[];
@@ -823,6 +826,10 @@ __global__ void __launch_bounds__(512) k_fused_half(double2* __restrict__ psi, c
     const int T = 1 << pd.b;
     const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 8: two lanes per row
     const uint64_t base = tile_base(blockIdx.x, pd);
+    // global offsets of the tile's high local bits (local index >> L), after the tile in LDS
+    uint64_t* hi_tab = reinterpret_cast<uint64_t*>(tile + T);
+    const int L = pd.L;
+    for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
     {
         // direct HBM -> LDS loads: LDS slot lh + tid receives element lh + swz(tid)
         const uint64_t g_sw = base | local_to_global(swz(tid), pd);
@@ -842,13 +849,20 @@ __global__ void __launch_bounds__(512) k_fused_half(double2* __restrict__ psi, c
         const int l0 = ins_bit(ins_bit(ins_bit(ins_bit(r, s0), s1), s2), s3);
         int a[8];
         hrow_slots(a, l0 | (h << s3), s0, s1, s2, std::make_integer_sequence<int, 8>{});
-        const uint64_t G0 = base | local_to_global(l0, pd);
+        const uint64_t G0 = base | (uint64_t)(l0 & ((1 << L) - 1)) | hi_tab[l0 >> L];
         double2 v[8];
         hgather(v, tile, a, std::make_integer_sequence<int, 8>{});
         for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_hop(v, ops[oi], G0, h, runs);
+        // the group's relabeling; bits without a parity mask are the same for every lane
         const uint32_t pc = pd.gpxc[gi];
-        const int xo = swz((px_bit(G0, pd.gpxm[gi][0], pc) << s0) | (px_bit(G0, pd.gpxm[gi][1], pc >> 1) << s1) |
-                           (px_bit(G0, pd.gpxm[gi][2], pc >> 2) << s2) | (px_bit(G0, pd.gpxm[gi][3], pc >> 3) << s3));
+        int px = 0;
+#pragma unroll
+        for (int j = 0; j < FUSE_GROUP_BITS; j++) {
+            const uint64_t m = pd.gpxm[gi][j];
+            const int bit = m ? px_bit(G0, m, pc >> j) : (int)((pc >> j) & 1u);
+            px |= bit << pd.gbits[gi][j];
+        }
+        const int xo = swz(px);
         // a row's two lanes are in one wavefront, whose LDS reads all precede its writes
         hscatter(v, tile, a, xo, std::make_integer_sequence<int, 8>{});
         __syncthreads();
@@ -1108,10 +1122,19 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
         if constexpr (NS == 1 && sizeof(R) == 8) {
-            if (s.split_rows && (pd.b == 11 || pd.b == 12)) {
-                if (lds > 65536) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 64 KiB");
+            if (s.split_rows && pd.b >= 11 && pd.b <= 13) {
+                const size_t lds_h = lds + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
+                if (lds_h > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 160 KiB of LDS");
+                if (lds_h > 65536) {
+                    static bool half_attr = false;
+                    if (!half_attr) {
+                        QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_half),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                        half_attr = true;
+                    }
+                }
                 KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
-                hipLaunchKernelGGL(k_fused_half, dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds, s.stream,
+                hipLaunchKernelGGL(k_fused_half, dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds_h, s.stream,
                                    reinterpret_cast<double2*>(psi), pd, reinterpret_cast<const FOpDev<double>*>(ops),
                                    druns);
                 QFVM_LAUNCHED();
