@@ -203,6 +203,16 @@ int qfvm_inner(qfvm_handle a, qfvm_handle b, double* re, double* im);
 int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n);
 
 /*
+ * Marginal probabilities of a set of distinct qubits: out[m] (2^n_qubits entries) is the sum of
+ * |amp_i|^2 over the basis states i whose bit qubits[j] equals bit j of m. Replaces the host-side
+ * reordering and partial trace of SimuResult.calc_probabilities (results.py:148-176, permutebits /
+ * ptrace in default_simulator.py:50-76); the Python layer orders `qubits` to give the reference's
+ * layout. Up to 12 qubits are reduced on the device (per-block histograms added in a fixed order);
+ * larger sets go through the full probability vector on the host.
+ */
+int qfvm_marginals(qfvm_handle h, const int32_t* qubits, int32_t n_qubits, double* out);
+
+/*
  * Sample basis-state indices, one per uniform u in [0,1). Replaces the per-shot CDF scan of
  * measure_samples (statevector.hpp:1380-1412): the result is the basis index whose cumulative
  * probability interval contains u, found by descending a pairwise sum tree of p_i = |amp_i|^2

@@ -67,6 +67,7 @@ SIGNATURES = {
     "qfvm_gather": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "qfvm_inner": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
     "qfvm_probabilities": (ctypes.c_int, [_vp, _vp, _u64]),
+    "qfvm_marginals": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
     "qfvm_sample": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "qfvm_norm2": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl)]),
     "qfvm_synchronize": (ctypes.c_int, [_vp]),
@@ -287,6 +288,13 @@ class DeviceState:
     def probabilities(self) -> np.ndarray:
         out = np.empty(1 << self.n, dtype=np.float64)
         check(lib().qfvm_probabilities(self._h, out.ctypes.data, out.size))
+        return out
+
+    def marginals(self, qubits) -> np.ndarray:
+        """out[m] = sum of |amp_i|^2 over i with bit qubits[j] of i equal to bit j of m."""
+        q = np.ascontiguousarray(qubits, dtype=np.int32)
+        out = np.empty(1 << q.size, dtype=np.float64)
+        check(lib().qfvm_marginals(self._h, q.ctypes.data if q.size else None, q.size, out.ctypes.data))
         return out
 
     def sample(self, uniforms) -> np.ndarray:

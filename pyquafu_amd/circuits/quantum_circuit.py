@@ -304,6 +304,38 @@ class QuantumCircuit:
         name = self.name if self.name else "Oracle"
         return CircuitWrapper(name, self, [] if qbits is None else qbits)
 
+    # ------------------------------------------------------------------ OpenQASM
+    def from_openqasm(self, openqasm: str):
+        """Replace this circuit by the OpenQASM 2.0 program (quantum_circuit.py:557-566,
+        qfasm_convertor.py:38-58): registers flattened in declaration order."""
+        from ..qfasm import parse_qasm  # pylint: disable=import-outside-toplevel
+
+        new = parse_qasm(openqasm)
+        self._num, self._cnum = new.num, new.cbits_num
+        self.instructions = new.instructions
+        self._gates = new.gates
+        self._measures = new._measures  # pylint: disable=protected-access
+        self._parameterized_gates = []
+        self.openqasm = openqasm
+        self.executable_on_backend = new.executable_on_backend
+        return self
+
+    def to_openqasm(self, with_para=False) -> str:
+        """OpenQASM 2.0 text of the gates and final measures (quantum_circuit.py:568-600)."""
+        valid = set(QuantumGate.gate_classes.keys()) | {"barrier", "delay", "reset"}
+        qasm = 'OPENQASM 2.0;\ninclude "qelib1.inc";\n'
+        qasm += f"qreg q[{self.num}];\n"
+        qasm += f"creg meas[{self.cbits_num}];\n"
+        for gate in self.gates:
+            if gate.name.lower() not in valid:
+                raise NotImplementedError(
+                    f"gate {gate.name} can not convert to qasm2 directly, you may decompose it manuallly")
+            qasm += gate.to_qasm(with_para) + ";\n"
+        for q, c in self.measures.items():
+            qasm += f"measure q[{q}] -> meas[{c}];\n"
+        self.openqasm = qasm
+        return qasm
+
     def unwrap(self):
         instructions, gates = [], []
         for op in self.instructions:

@@ -736,6 +736,19 @@ int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n) {
     });
 }
 
+int qfvm_marginals(qfvm_handle h, const int32_t* qubits, int32_t n_qubits, double* out) {
+    return guarded([&] {
+        State* s = need(h);
+        if (n_qubits < 0 || n_qubits > s->n || n_qubits > 30) throw Error(QFVM_ERR_VALUE, "bad qubit count");
+        if (!out || (n_qubits > 0 && !qubits)) throw Error(QFVM_ERR_VALUE, "null buffer");
+        std::vector<int> q(qubits, qubits + n_qubits);
+        check_qubits(*s, q);
+        std::vector<double> probs;
+        reduce_marginal(*s, q, probs);
+        std::memcpy(out, probs.data(), probs.size() * sizeof(double));
+    });
+}
+
 int qfvm_sample(qfvm_handle h, const double* uniforms, int64_t shots, uint64_t* out_index) {
     return guarded([&] {
         State* s = need(h);

@@ -73,6 +73,9 @@ class SimuResult(Result):
             self._probabilities = np.zeros(2**num)
             for key, v in counts.items():
                 self._probabilities[int(key, 2)] = v / total
+        elif self._meta_data.get("probabilities") is not None:
+            # measured-qubit marginals reduced on the device by the simulator, same layout
+            self._probabilities = np.asarray(self._meta_data["probabilities"])
         elif "statevector" in self._meta_data:
             psi = self.get_statevector()
             measures = list(self["measures"].keys())

@@ -22,8 +22,10 @@ def simulate(
 
     "statevector" and "noisy statevetor" (trajectories, the reference's spelling) run on the
     engine; the reference's "clifford" simulator is another engine and is not provided. `precision` ("c128" default, or "c64") and `seed` are extensions."""
-    if isinstance(qc, str):
-        raise NotImplementedError("OpenQASM input is not supported by this engine; build a QuantumCircuit")
+    if isinstance(qc, str):  # OpenQASM 2.0 text (simulators/__init__.py:52-57)
+        from ..qfasm import parse_qasm  # pylint: disable=import-outside-toplevel
+
+        qc = parse_qasm(qc)
     if simulator == "statevector":
         from .simulator import SVSimulator  # pylint: disable=import-outside-toplevel
 

@@ -18,6 +18,23 @@ from .program import KRAUS, UCHAN, compile_gates
 from .qfvm import _call, applyop_statevec, bitstring_counts, expect_statevec, simulate_circuit
 
 
+MAX_DEVICE_MARGINAL = 12  # measured qubits reduced on the device while the final state is resident
+
+
+def measured_marginals(qc, precision: str):
+    """Probabilities over the measured qubits in SimuResult.calc_probabilities' layout
+    (results.py:148-176): index bit M-1-j <-> the qubit with the j-th smallest cbit. Taken from the
+    final state still on the device after simulate_circuit (the engine pool keeps one state per
+    qubit count and dtype); None when nothing is measured or too many qubits are."""
+    meas = qc.measures
+    if not meas or len(meas) > MAX_DEVICE_MARGINAL:
+        return None
+    qubits, cbits = list(meas.keys()), list(meas.values())
+    by_cbit = [qubits[k] for k in np.argsort(cbits)]
+    dev = _call(_native.pool.get, qc.num, precision)
+    return _call(dev.marginals, by_cbit[::-1])
+
+
 class Simulator:
     def run(self, *args, **kwargs):
         raise NotImplementedError
@@ -62,6 +79,9 @@ class SVSimulator(Simulator):
         count_dict, psi = simulate_circuit(qc, psi, shots, seed=self.seed, dtype=self.precision)
         res_info["statevector"] = psi
         res_info["counts"] = count_dict
+        marginals = measured_marginals(qc, self.precision)
+        if marginals is not None:
+            res_info["probabilities"] = marginals
         if hamiltonian:
             paulis = hamiltonian.paulis
             res = expect_statevec(res_info["statevector"], paulis)

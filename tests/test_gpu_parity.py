@@ -430,3 +430,34 @@ def test_diagonal_runs_match_oracle(n, dtype, kernel):
     dev.close()
     tol = 1e-12 if dtype == "c128" else 2e-5
     assert np.abs(got - cpu.store()).max() < tol
+
+
+# ------------------------------------------------------------------ marginal probabilities
+@pytest.mark.parametrize("n,k,dtype", [(5, 1, "c128"), (9, 4, "c128"), (14, 12, "c128"), (16, 7, "c64"), (13, 13, "c128")])
+def test_marginals_match_numpy(n, k, dtype):
+    """qfvm_marginals over k qubits in arbitrary order (13 goes through the host fallback)."""
+    rng = np.random.Generator(np.random.PCG64(2300 + n * 10 + k))
+    psi = rand_state(n, rng)
+    qubits = [int(q) for q in rng.permutation(n)[:k]]
+    dev = gpu_state(n, dtype)
+    dev.load(psi)
+    got = dev.marginals(qubits)
+    want = dense.marginal(dev.store(), n, qubits)
+    assert np.abs(got - want).max() < (1e-14 if dtype == "c128" else 1e-7)
+    assert abs(got.sum() - 1.0) < (1e-12 if dtype == "c128" else 1e-5)
+
+
+def test_result_probabilities_use_device_marginals():
+    """SimuResult.probabilities from the device marginals equals the reference's host route
+    (reverse the qubit order, partial trace over the measured qubits, order by cbit)."""
+    from pyquafu_amd.results.results import SimuResult
+
+    rng = np.random.Generator(np.random.PCG64(2400))
+    n = 10
+    qc = workloads.random_u3_cx(n, 3, 2, seed=2401, measure=False)
+    qc.measure([7, 2, 9, 4], [2, 0, 3, 1])
+    res = pq.simulate(qc, shots=10)
+    assert "probabilities" in res._meta_data  # pylint: disable=protected-access
+    host = SimuResult({k: v for k, v in res._meta_data.items() if k != "probabilities"})  # pylint: disable=protected-access
+    assert np.abs(res.probabilities - host.probabilities).max() < 1e-14
+    del rng
