@@ -812,64 +812,94 @@ __device__ __forceinline__ void hgather(double2 (&v)[8], const double2* tile, co
     ((v[Ms] = tile[a[Ms]]), ...);
 }
 
+// the slots are recomputed from the row's swizzled base (slot ^ relabel offset), so no address
+// register has to stay live across the op loop
 template <int... Ms>
-__device__ __forceinline__ void hscatter(const double2 (&v)[8], double2* tile, const int (&a)[8], int xo,
+__device__ __forceinline__ void hscatter(const double2 (&v)[8], double2* tile, int sx, int s0, int s1, int s2,
                                          std::integer_sequence<int, Ms...>) {
-    ((tile[a[Ms] ^ xo] = v[Ms]), ...);
+    ((tile[sx ^ swz(((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2))] = v[Ms]), ...);
 }
 
-__global__ void __launch_bounds__(1024) k_fused_half(double2* __restrict__ psi, const PassDesc pd,
-                                                     const FOpDev<double>* __restrict__ ops,
-                                                     const unsigned char* __restrict__ runs) {
+// NS = 2: the reverse sweep, both states' tiles side by side in LDS (as k_fused_pass), gradient
+// stages evaluated from LDS between compute groups.
+// (two-state tiles fit two workgroups per CU at most, so that instantiation may use 256 registers)
+template <int NS>
+__global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __restrict__ psi, double2* __restrict__ psi1,
+                                                     const PassDesc pd, const FOpDev<double>* __restrict__ ops,
+                                                     const unsigned char* __restrict__ runs,
+                                                     double* __restrict__ partials, uint64_t tiles) {
     extern __shared__ unsigned char lds_raw[];
     double2* tile = reinterpret_cast<double2*>(lds_raw);
     const int T = 1 << pd.b;
     const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 8: two lanes per row
     const uint64_t base = tile_base(blockIdx.x, pd);
-    // global offsets of the tile's high local bits (local index >> L), after the tile in LDS
-    uint64_t* hi_tab = reinterpret_cast<uint64_t*>(tile + T);
+    // after the tiles: the gradient stages' wavefront sums (NS = 2), then the global offsets of the
+    // tile's high local bits (local index >> L)
+    double* red = reinterpret_cast<double*>(tile + NS * T);
+    uint64_t* hi_tab = reinterpret_cast<uint64_t*>(red + (NS == 2 ? 64 : 0));
     const int L = pd.L;
     for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
     {
         // direct HBM -> LDS loads: LDS slot lh + tid receives element lh + swz(tid)
         const uint64_t g_sw = base | local_to_global(swz(tid), pd);
         const int wave = tid >> 6;
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const double2* src_state = st ? psi1 : psi;
 #pragma unroll 8
-        for (int lh = 0; lh < T; lh += NT) {
-            const void* src = (const void*)(psi + (g_sw | local_to_global(lh, pd)));
-            auto* dst = (__attribute__((address_space(3))) void*)(tile + lh + wave * 64);
-            __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+            for (int lh = 0; lh < T; lh += NT) {
+                const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
+                auto* dst = (__attribute__((address_space(3))) void*)(tile + st * T + lh + wave * 64);
+                __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+            }
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const int h = tid & 1, r = tid >> 1;
     for (int gi = 0; gi < pd.ngroups; gi++) {
+        if constexpr (NS == 2) {
+            if ((pd.stage_mask >> gi) & 1u) {
+                grad_stage<double>(tile, tile + T, T, base | local_to_global(tid, pd), pd, ops + pd.gop[gi],
+                                   pd.gop[gi + 1] - pd.gop[gi], red, partials, tiles);
+                continue;  // grad_stage ends with a barrier
+            }
+        }
         const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];
         const int l0 = ins_bit(ins_bit(ins_bit(ins_bit(r, s0), s1), s2), s3);
-        int a[8];
-        hrow_slots(a, l0 | (h << s3), s0, s1, s2, std::make_integer_sequence<int, 8>{});
+        const int sl = swz(l0 | (h << s3));
         const uint64_t G0 = base | (uint64_t)(l0 & ((1 << L) - 1)) | hi_tab[l0 >> L];
-        double2 v[8];
-        hgather(v, tile, a, std::make_integer_sequence<int, 8>{});
-        for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_hop(v, ops[oi], G0, h, runs);
-        // the group's relabeling; bits without a parity mask are the same for every lane
-        const uint32_t pc = pd.gpxc[gi];
-        int px = 0;
+        for (int st = 0; st < NS; st++) {
+            double2* tl = tile + st * T;
+            double2 v[8];
+            {
+                int a[8];
+                hrow_slots(a, l0 | (h << s3), s0, s1, s2, std::make_integer_sequence<int, 8>{});
+                hgather(v, tl, a, std::make_integer_sequence<int, 8>{});
+            }
+            for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_hop(v, ops[oi], G0, h, runs);
+            // the group's relabeling; bits without a parity mask are the same for every lane
+            const uint32_t pc = pd.gpxc[gi];
+            int px = 0;
 #pragma unroll
-        for (int j = 0; j < FUSE_GROUP_BITS; j++) {
-            const uint64_t m = pd.gpxm[gi][j];
-            const int bit = m ? px_bit(G0, m, pc >> j) : (int)((pc >> j) & 1u);
-            px |= bit << pd.gbits[gi][j];
+            for (int j = 0; j < FUSE_GROUP_BITS; j++) {
+                const uint64_t m = pd.gpxm[gi][j];
+                const int bit = m ? px_bit(G0, m, pc >> j) : (int)((pc >> j) & 1u);
+                px |= bit << pd.gbits[gi][j];
+            }
+            // a row's two lanes are in one wavefront, whose LDS reads all precede its writes
+            hscatter(v, tl, sl ^ swz(px), s0, s1, s2, std::make_integer_sequence<int, 8>{});
         }
-        const int xo = swz(px);
-        // a row's two lanes are in one wavefront, whose LDS reads all precede its writes
-        hscatter(v, tile, a, xo, std::make_integer_sequence<int, 8>{});
         __syncthreads();
     }
     const uint64_t g_tid = base | local_to_global(tid, pd);
+#pragma unroll
+    for (int st = 0; st < NS; st++) {
+        double2* dst_state = st ? psi1 : psi;
+        const double2* tl = tile + st * T;
 #pragma unroll 8
-    for (int lh = 0; lh < T; lh += NT) psi[g_tid | local_to_global(lh, pd)] = tile[swz(lh + tid)];
+        for (int lh = 0; lh < T; lh += NT) dst_state[g_tid | local_to_global(lh, pd)] = tl[swz(lh + tid)];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1121,25 +1151,34 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         }
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
-        if constexpr (NS == 1 && sizeof(R) == 8) {
-            if (s.split_rows && pd.b >= 11 && pd.b <= 13) {
+        // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
+        // between passes stages its matrix there)
+        double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;
+        if constexpr (sizeof(R) == 8) {
+            const int hb_max = NS == 1 ? 13 : 12;  // at most 1024 / 512 threads
+            if (s.split_rows && pd.b >= 11 && pd.b <= hb_max) {
                 const size_t lds_h = lds + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
                 if (lds_h > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 160 KiB of LDS");
                 if (lds_h > 65536) {
                     static bool half_attr = false;
                     if (!half_attr) {
-                        QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_half),
+                        QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_half<NS>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
                         half_attr = true;
                     }
                 }
-                KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
-                hipLaunchKernelGGL(k_fused_half, dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds_h, s.stream,
-                                   reinterpret_cast<double2*>(psi), pd, reinterpret_cast<const FOpDev<double>*>(ops),
-                                   druns);
-                QFVM_LAUNCHED();
+                {
+                    KernelScope ks(s, NS == 1 ? "fused_pass" : "fused_pass2", 2.0 * NS * (double)s.size * sizeof(V2<R>));
+                    hipLaunchKernelGGL((k_fused_half<NS>), dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds_h,
+                                       s.stream, reinterpret_cast<double2*>(psi), reinterpret_cast<double2*>(psi1), pd,
+                                       reinterpret_cast<const FOpDev<double>*>(ops), druns, partials, tiles);
+                    QFVM_LAUNCHED();
+                }
+                if (cols > 0) launch_column_sums(s, partials, tiles, cols, slot_of, grads_dev);
                 continue;
             }
+        }
+        if constexpr (NS == 1 && sizeof(R) == 8) {
             if (s.stream_tiles && pd.b >= 11 && pd.b <= 13 && pd.ngroups >= 1 && pd.ngroups <= FUSE_MAX_GROUPS) {
                 const unsigned grid = stream_grid(s, lds, threads, tiles);
                 KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
@@ -1150,9 +1189,6 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
                 continue;
             }
         }
-        // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
-        // between passes stages its matrix there)
-        double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;
         {
             KernelScope ks(s, NS == 1 ? "fused_pass" : "fused_pass2", 2.0 * NS * (double)s.size * sizeof(V2<R>));
             hipLaunchKernelGGL((k_fused_pass<R, NS>), dim3((unsigned)tiles), dim3(threads), lds, s.stream, psi, psi1,

@@ -52,6 +52,21 @@ def test_adjoint_tile_shapes(tile, low, groups):
     assert np.abs(got - want).max() < 1e-10
 
 
+@pytest.mark.parametrize("split,tile", [(1, 12), (1, 13), (0, 12), (0, 13)])
+def test_adjoint_kernel_variants(split, tile):
+    """The reverse sweep on the split-row two-state kernel (two lanes per row; option tile_bits = b + 1
+    for two-state passes) and on the one-lane kernel, against the oracle."""
+    n = 15
+    rng = np.random.Generator(np.random.PCG64(1350 + 10 * split + tile))
+    make, thetas, ops, gens = rotation_program(n, 90, rng)
+    terms, coeffs = random_pauli_terms(n, 6, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, options=[("split_rows", split), ("tile_bits", tile)])
+    assert abs(e - e_want) < 1e-11
+    assert np.abs(got - want).max() < 1e-10
+
+
 def test_adjoint_complex64():
     n = 10
     rng = np.random.Generator(np.random.PCG64(1400))
