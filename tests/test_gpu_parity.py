@@ -461,3 +461,39 @@ def test_result_probabilities_use_device_marginals():
     host = SimuResult({k: v for k, v in res._meta_data.items() if k != "probabilities"})  # pylint: disable=protected-access
     assert np.abs(res.probabilities - host.probabilities).max() < 1e-14
     del rng
+
+
+# ------------------------------------------------------------------ full-size properties
+def test_full_size_qft_closed_form_30_qubits():
+    """The 30-qubit QFT of a basis state against its closed form on 100,000 sampled amplitudes
+    (complex128 and complex64), plus the norm."""
+    n, x = 30, 0x2B5E_71C3 & ((1 << 30) - 1)
+    prog = compile_circuit(workloads.qft(n, x))
+    ys = np.random.Generator(np.random.PCG64(3000)).integers(0, 1 << n, size=100000, dtype=np.int64)
+    want = workloads.qft_amplitudes(n, x, ys)
+    for dtype, tol in (("c128", 1e-15), ("c64", 2e-10)):
+        dev = _native.DeviceState(n, dtype)
+        try:
+            dev.apply(prog)
+            got = dev.gather(ys)
+            assert np.abs(got - want).max() < tol, dtype
+            assert abs(dev.norm2() - 1.0) < (1e-12 if dtype == "c128" else 1e-5)
+        finally:
+            dev.close()
+
+
+def test_full_size_round_trip_30_qubits():
+    """The 30-qubit benchmark circuit (2,960 gates) followed by its inverse returns |0...0>: a
+    size-independent check of every fused pass at the benchmark's shape."""
+    n = 30
+    qc = workloads.random_u3_cx(n, 40, 15, seed=20250002, measure=False)
+    fwd, back = compile_circuit(qc), compile_circuit(qc.dagger())
+    dev = _native.DeviceState(n, "c128")
+    try:
+        dev.apply(fwd)
+        dev.apply(back)
+        a0 = dev.gather([0])[0]
+        assert abs(a0 - 1.0) < 1e-11
+        assert abs(dev.norm2() - 1.0) < 1e-11
+    finally:
+        dev.close()
