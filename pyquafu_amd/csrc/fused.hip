@@ -48,7 +48,8 @@ struct RunTerm {
     uint64_t cl;    // index bits outside the group that must all be set
     uint64_t pm;    // selector: v1 where parity(G0 & pm) ^ pc is 1, else v0
     uint32_t pc;
-    int32_t angle;  // 1: unit-modulus values given as angles (v[0] = arg v0, v[1] = arg v1)
+    int32_t angle;  // 1: unit-modulus values given as angles (v[0] = arg v0, v[1] = arg v1);
+                    // 2: linear, the angle v[0] * (G0 & cl) (cl a set of index bits, no condition)
     R v[4];         // v0 (re, im), v1 (re, im)
 };
 template <typename R>
@@ -242,8 +243,12 @@ __device__ __forceinline__ void run_subset(V2<R> (&v)[16], const RunHead<R>* __r
     const int t1 = rh->tbeg[S + 1];
     for (int t = rh->tbeg[S]; t < t1; t++) {
         const RunTerm<R>& tm = terms[t];
+        if (tm.angle == 2) {
+            th += tm.v[0] * (R)(G0 & tm.cl);
+            continue;
+        }
         const bool cond = (G0 & tm.cl) == tm.cl;
-        const bool sel = px_bit(G0, tm.pm, tm.pc);
+        const bool sel = tm.pm ? (bool)px_bit(G0, tm.pm, tm.pc) : (bool)(tm.pc & 1u);
         if (tm.angle) {
             th += cond ? (sel ? tm.v[1] : tm.v[0]) : (R)0;
         } else {
@@ -734,8 +739,12 @@ __device__ __forceinline__ void hrun_subset(double2 (&v)[8], const RunHead<doubl
     const int t1 = rh->tbeg[S + 1];
     for (int t = rh->tbeg[S]; t < t1; t++) {
         const RunTerm<double>& tm = terms[t];
+        if (tm.angle == 2) {
+            th += tm.v[0] * (double)(G0 & tm.cl);
+            continue;
+        }
         const bool cond = (G0 & tm.cl) == tm.cl;
-        const bool sel = px_bit(G0, tm.pm, tm.pc);
+        const bool sel = tm.pm ? (bool)px_bit(G0, tm.pm, tm.pc) : (bool)(tm.pc & 1u);
         if (tm.angle) {
             th += cond ? (sel ? tm.v[1] : tm.v[0]) : 0.0;
         } else {
@@ -912,6 +921,7 @@ static FOpDev<R> encode_run(const FGroup& G, int run, std::vector<unsigned char>
         uint64_t cl, pm;
         uint32_t pc;
         cd v0, v1;
+        double scale = 0.0;  // linear terms: the angle is scale * (G0 & cl)
     };
     cd f0[16];
     for (int S = 0; S < 16; S++) f0[S] = 1.0;
@@ -937,10 +947,50 @@ static FOpDev<R> encode_run(const FGroup& G, int run, std::vector<unsigned char>
             add(C | (1 << o.g), o.cm_ext, 0, 0, d1 / d0, d1 / d0);
         }
     }
+    auto unit = [](cd z) { return std::abs(std::abs(z) - 1.0) < 1e-14; };
+    // Unit-modulus terms of one subset that each depend on a single index bit k (no relabel
+    // selector) with angle a_k = c * 2^k for a common c -- the controlled phases of a QFT -- add up
+    // to c * (G0 & M), M the union of their bits: one linear term replaces them. c agrees to 1e-15
+    // relative between the merged terms, so each angle moves by at most that much.
+    {
+        std::vector<HT> kept, linear;
+        std::vector<char> used(terms.size(), 0);
+        for (size_t a = 0; a < terms.size(); a++) {
+            const HT& ta = terms[a];
+            auto eligible = [&](const HT& t) {
+                return t.pm == 0 && __builtin_popcountll(t.cl) == 1 && unit(t.v0) && unit(t.v1);
+            };
+            if (used[a] || !eligible(ta)) continue;
+            auto coef = [&](const HT& t) {
+                return std::arg(t.pc ? t.v1 : t.v0) / std::ldexp(1.0, __builtin_ctzll(t.cl));
+            };
+            const double c = coef(ta);
+            uint64_t mask = 0;
+            std::vector<size_t> members;
+            for (size_t b = a; b < terms.size(); b++) {
+                const HT& tb = terms[b];
+                if (used[b] || tb.S != ta.S || !eligible(tb) || (mask & tb.cl)) continue;
+                const double cb = coef(tb);
+                if (std::abs(cb - c) <= 1e-15 * std::abs(c) || (c == 0.0 && cb == 0.0)) {
+                    members.push_back(b);
+                    mask |= tb.cl;
+                }
+            }
+            if (members.size() < 2) continue;
+            for (size_t b : members) used[b] = 1;
+            HT lt = ta;
+            lt.cl = mask;
+            lt.scale = c;
+            linear.push_back(lt);
+        }
+        for (size_t a = 0; a < terms.size(); a++)
+            if (!used[a]) kept.push_back(terms[a]);
+        kept.insert(kept.end(), linear.begin(), linear.end());
+        terms.swap(kept);
+    }
     std::stable_sort(terms.begin(), terms.end(), [](const HT& a, const HT& b) { return a.S < b.S; });
     RunHead<R> head{};
     uint32_t active = 0, angles = 0;
-    auto unit = [](cd z) { return std::abs(std::abs(z) - 1.0) < 1e-14; };
     for (int S = 0; S < 16; S++) {
         head.f0[2 * S] = (R)f0[S].real();
         head.f0[2 * S + 1] = (R)f0[S].imag();
@@ -962,7 +1012,11 @@ static FOpDev<R> encode_run(const FGroup& G, int run, std::vector<unsigned char>
         rt.cl = terms[k].cl;
         rt.pm = terms[k].pm;
         rt.pc = terms[k].pc;
-        if (unit(terms[k].v0) && unit(terms[k].v1)) {
+        if (terms[k].scale != 0.0) {
+            rt.angle = 2;
+            rt.v[0] = (R)terms[k].scale;
+            angles |= 1u << terms[k].S;
+        } else if (unit(terms[k].v0) && unit(terms[k].v1)) {
             rt.angle = 1;
             rt.v[0] = (R)std::arg(terms[k].v0);
             rt.v[1] = (R)std::arg(terms[k].v1);
