@@ -13,6 +13,7 @@
 // load loop's unit stride both spread across banks.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "fused.h"
 #include "sincos.h"
@@ -1115,16 +1116,29 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
     return pd;
 }
 
+// Raises a kernel's dynamic LDS limit to 160 KiB, once per (kernel, device); handles run on several
+// host threads.
+static void allow_large_lds(const void* fn, int device) {
+    static std::mutex lock;
+    static std::vector<std::pair<const void*, int>> done;
+    std::lock_guard<std::mutex> guard(lock);
+    for (const auto& d : done)
+        if (d.first == fn && d.second == device) return;
+    QFVM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    done.emplace_back(fn, device);
+}
+
 // Workgroups of a persistent k_fused_stream launch: as many as fit on the device at once.
 static unsigned stream_grid(const State& s, size_t lds, unsigned threads, uint64_t tiles) {
     static std::vector<std::pair<uint64_t, unsigned>> known;  // (device, lds, threads) -> grid
+    static std::mutex known_lock;                              // handles run on several host threads
+    std::lock_guard<std::mutex> guard(known_lock);
     const uint64_t key = ((uint64_t)s.device << 40) | ((uint64_t)lds << 12) | threads;
     unsigned grid = 0;
     for (const auto& kv : known)
         if (kv.first == key) grid = kv.second;
     if (grid == 0) {
-        QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_stream),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        allow_large_lds(reinterpret_cast<const void*>(&k_fused_stream), s.device);
         int per_cu = 0, cus = 0;
         QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused_stream),
                                                               (int)threads, lds));
@@ -1195,14 +1209,7 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         k++;
         const size_t lds = NS * ((size_t)1 << pd.b) * sizeof(V2<R>) + (NS == 2 ? 64 * sizeof(double) : 0);
         if (lds > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "fused pass tiles exceed 160 KiB of LDS");
-        if (lds > 65536) {
-            static bool attr_set = false;
-            if (!attr_set) {
-                QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<R, NS>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                attr_set = true;
-            }
-        }
+        if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_fused_pass<R, NS>), s.device);
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
         // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
@@ -1213,14 +1220,7 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
             if (s.split_rows && pd.b >= 11 && pd.b <= hb_max) {
                 const size_t lds_h = lds + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
                 if (lds_h > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 160 KiB of LDS");
-                if (lds_h > 65536) {
-                    static bool half_attr = false;
-                    if (!half_attr) {
-                        QFVM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_half<NS>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                        half_attr = true;
-                    }
-                }
+                if (lds_h > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_fused_half<NS>), s.device);
                 {
                     KernelScope ks(s, NS == 1 ? "fused_pass" : "fused_pass2", 2.0 * NS * (double)s.size * sizeof(V2<R>));
                     hipLaunchKernelGGL((k_fused_half<NS>), dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds_h,
