@@ -843,8 +843,8 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
             lower_op(tmp, prog, prog->ops[k], gates);
         }
         FuseConfig cfg;
-        cfg.tile_bits = tile_bits > 0 ? tile_bits : (tmp.dtype == QFVM_C128 ? 12 : 13);
-        cfg.low_bits = low_bits > 0 ? low_bits : 6;
+        cfg.tile_bits = tile_bits > 0 ? tile_bits : default_tile_bits(tmp.dtype, 1);
+        cfg.low_bits = low_bits > 0 ? low_bits : default_low_bits(tmp.dtype, 1);
         cfg.max_groups = max_groups > 0 ? max_groups : 12;
         cfg.max_cost = pass_cost > 0 ? pass_cost : 0;
         std::vector<PlanItem> plan = plan_gates(gates, n, cfg);

@@ -563,19 +563,20 @@ __global__ void __launch_bounds__(512) k_fused_stream(double2* __restrict__ psi,
 
 // ------------------------------------------------------------------------------------------------
 // Split-row variant (one-state complex128 passes of 2^11 .. 2^13 amplitudes): two lanes (lane,
-// lane ^ 1) share each 16-member row; lane bit 0 = h holds the members with group bit 3 equal to h,
+// lane ^ 8) share each 16-member row; lane bit 3 = h holds the members with group bit 3 equal to h,
 // register m <-> member (h << 3) | m. A tile then has 2^(b-3) threads, twice the waves of
 // k_fused_pass, to stream it in and out, and a thread holds 8 amplitudes instead of 16. Gates on
 // group bits 0..2 stay lane-local; a gate on group bit 3 pairs register m of the two lanes: the
-// partner's value comes over a DPP quad permutation and each lane computes its own output with
+// partner's value comes over a DPP row rotation by 8 and each lane computes its own output with
 // per-lane coefficients (m00, m01 where h = 0; m11, m10 where h = 1). Controls on group bit 3 and
 // the diagonal-run subsets that contain it become per-lane conditions.
 
-// the partner lane's (lane ^ 1) copy of a value; every lane of the wavefront must be active
+// the partner lane's (lane ^ 8) copy of a value: DPP row_ror:8 (0x128) rotates each 16-lane row by
+// 8; every lane of the wavefront must be active
 __device__ __forceinline__ double xlane(double x) {
     const uint64_t u = __builtin_bit_cast(uint64_t, x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0xB1, 0xF, 0xF, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0xB1, 0xF, 0xF, false);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x128, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x128, 0xF, 0xF, false);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 __device__ __forceinline__ double2 xlane(double2 v) { return make_double2(xlane(v.x), xlane(v.y)); }
@@ -866,7 +867,10 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    const int h = tid & 1, r = tid >> 1;
+    // the two lanes of a row are lane bit 3 apart, so the 8 lanes that one LDS cycle of a 16-byte
+    // access serves (lanes 8k .. 8k+7) hold 8 consecutive rows, which the swizzle puts in distinct
+    // banks (lane bit 0 pairing put both halves of 4 rows there: 2-way bank conflicts)
+    const int h = (tid >> 3) & 1, r = (tid & 7) | ((tid >> 4) << 3);
     for (int gi = 0; gi < pd.ngroups; gi++) {
         if constexpr (NS == 2) {
             if ((pd.stage_mask >> gi) & 1u) {

@@ -293,11 +293,10 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
 
 static FuseConfig config_of(const State& s, int states) {
     FuseConfig cfg;
-    // the two-state sweep holds both tiles in LDS, so its default tile is half as large
-    const int def = (s.dtype == QFVM_C128 ? 12 : 13) - (states - 1);
-    cfg.tile_bits = s.tile_bits > 0 ? s.tile_bits - (states - 1) : def;
+    // the two-state sweep holds both tiles in LDS, so an explicit tile size is halved for it
+    cfg.tile_bits = s.tile_bits > 0 ? s.tile_bits - (states - 1) : default_tile_bits(s.dtype, states);
     cfg.tile_bits = std::max(FUSE_GROUP_BITS, std::min(cfg.tile_bits, FUSE_MAX_TILE_BITS));
-    cfg.low_bits = s.low_bits > 0 ? s.low_bits : 6;
+    cfg.low_bits = s.low_bits > 0 ? s.low_bits : default_low_bits(s.dtype, states);
     cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
     cfg.max_cost = s.pass_cost;
     return cfg;
