@@ -579,13 +579,22 @@ __device__ __forceinline__ double xlane(double x) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x128, 0xF, 0xF, false);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
-__device__ __forceinline__ double2 xlane(double2 v) { return make_double2(xlane(v.x), xlane(v.y)); }
+__device__ __forceinline__ float xlane(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x128, 0xF, 0xF, false));
+}
+template <typename R>
+__device__ __forceinline__ V2<R> xlane_v(V2<R> v) {
+    return mk<R>(xlane(v.x), xlane(v.y));
+}
+// bit pattern of one component (relabel flushes blend registers bitwise)
+template <typename R>
+using UBits = std::conditional_t<sizeof(R) == 8, uint64_t, uint32_t>;
 
 // 2x2 kinds on a lane-local group bit G < 3 (pairs m, m | 1 << G) with group controls cm7 on bits 0..2
 // (CTRL = false: no control among bits 0..2, no per-pair tests)
-template <int KIND, int G, int M, bool CTRL>
-__device__ __forceinline__ void hpair_local(double2 (&v)[8], int cm7, double2 m00, double2 m01, double2 m10,
-                                            double2 m11) {
+template <typename R, int KIND, int G, int M, bool CTRL>
+__device__ __forceinline__ void hpair_local(V2<R> (&v)[8], int cm7, V2<R> m00, V2<R> m01, V2<R> m10,
+                                            V2<R> m11) {
     if constexpr ((M >> G) & 1) {
         return;
     } else {
@@ -593,217 +602,218 @@ __device__ __forceinline__ void hpair_local(double2 (&v)[8], int cm7, double2 m0
             if ((M & cm7) != cm7) return;
         }
         constexpr int M1 = M | (1 << G);
-        const double2 a = v[M], b = v[M1];
+        const V2<R> a = v[M], b = v[M1];
         if constexpr (KIND == F_SWAP) {
             v[M] = b;
             v[M1] = a;
         } else if constexpr (KIND == F_DIAG) {
-            v[M] = cmul<double>(m00, a);
-            v[M1] = cmul<double>(m11, b);
+            v[M] = cmul<R>(m00, a);
+            v[M1] = cmul<R>(m11, b);
         } else if constexpr (KIND == F_REAL) {
-            v[M] = mk<double>(m00.x * a.x + m01.x * b.x, m00.x * a.y + m01.x * b.y);
-            v[M1] = mk<double>(m10.x * a.x + m11.x * b.x, m10.x * a.y + m11.x * b.y);
+            v[M] = mk<R>(m00.x * a.x + m01.x * b.x, m00.x * a.y + m01.x * b.y);
+            v[M1] = mk<R>(m10.x * a.x + m11.x * b.x, m10.x * a.y + m11.x * b.y);
         } else {
-            v[M] = cmul2<double>(m00, a, m01, b);
-            v[M1] = cmul2<double>(m10, a, m11, b);
+            v[M] = cmul2<R>(m00, a, m01, b);
+            v[M1] = cmul2<R>(m10, a, m11, b);
         }
     }
 }
 
 // 2x2 kinds on group bit 3 (across the lane pair): own coefficient c0, partner coefficient c1
-template <int KIND, int M, bool CTRL>
-__device__ __forceinline__ void hpair_cross(double2 (&v)[8], int cm7, double2 c0, double2 c1) {
+template <typename R, int KIND, int M, bool CTRL>
+__device__ __forceinline__ void hpair_cross(V2<R> (&v)[8], int cm7, V2<R> c0, V2<R> c1) {
     if constexpr (CTRL) {
         if ((M & cm7) != cm7) return;
     }
     if constexpr (KIND == F_DIAG) {
-        v[M] = cmul<double>(c0, v[M]);
+        v[M] = cmul<R>(c0, v[M]);
     } else if constexpr (KIND == F_SWAP) {
-        v[M] = xlane(v[M]);
+        v[M] = xlane_v<R>(v[M]);
     } else {
-        const double2 o = xlane(v[M]);
+        const V2<R> o = xlane_v<R>(v[M]);
         if constexpr (KIND == F_REAL)
-            v[M] = mk<double>(c0.x * v[M].x + c1.x * o.x, c0.x * v[M].y + c1.x * o.y);
+            v[M] = mk<R>(c0.x * v[M].x + c1.x * o.x, c0.x * v[M].y + c1.x * o.y);
         else
-            v[M] = cmul2<double>(c0, v[M], c1, o);
+            v[M] = cmul2<R>(c0, v[M], c1, o);
     }
 }
 
-template <int KIND, int G, int... Ms>
-__device__ __forceinline__ void hpairs(double2 (&v)[8], int cm7, double2 m00, double2 m01, double2 m10, double2 m11,
+template <typename R, int KIND, int G, int... Ms>
+__device__ __forceinline__ void hpairs(V2<R> (&v)[8], int cm7, V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11,
                                        std::integer_sequence<int, Ms...>) {
     if constexpr (G < 3) {
         if (cm7 == 0)
-            (hpair_local<KIND, G, Ms, false>(v, cm7, m00, m01, m10, m11), ...);
+            (hpair_local<R, KIND, G, Ms, false>(v, cm7, m00, m01, m10, m11), ...);
         else
-            (hpair_local<KIND, G, Ms, true>(v, cm7, m00, m01, m10, m11), ...);
+            (hpair_local<R, KIND, G, Ms, true>(v, cm7, m00, m01, m10, m11), ...);
     } else {  // m00 / m01 carry this lane's own / partner coefficients
         if (cm7 == 0)
-            (hpair_cross<KIND, Ms, false>(v, cm7, m00, m01), ...);
+            (hpair_cross<R, KIND, Ms, false>(v, cm7, m00, m01), ...);
         else
-            (hpair_cross<KIND, Ms, true>(v, cm7, m00, m01), ...);
+            (hpair_cross<R, KIND, Ms, true>(v, cm7, m00, m01), ...);
     }
 }
 
-template <int KIND, int G>
-__device__ __forceinline__ void hop_2x2(double2 (&v)[8], const FOpDev<double>& o, bool ok, uint64_t G0, int h) {
-    double2 m00 = mk<double>(o.m[0], o.m[1]), m01 = mk<double>(o.m[2], o.m[3]);
-    double2 m10 = mk<double>(o.m[4], o.m[5]), m11 = mk<double>(o.m[6], o.m[7]);
+template <typename R, int KIND, int G>
+__device__ __forceinline__ void hop_2x2(V2<R> (&v)[8], const FOpDev<R>& o, bool ok, uint64_t G0, int h) {
+    V2<R> m00 = mk<R>(o.m[0], o.m[1]), m01 = mk<R>(o.m[2], o.m[3]);
+    V2<R> m10 = mk<R>(o.m[4], o.m[5]), m11 = mk<R>(o.m[6], o.m[7]);
     if (o.flags & FL_TARGET_PX) {  // registers hold the pair swapped where the relabel bit is set
         const bool sw = px_bit(G0, o.pxmask, (o.flags >> 1) & 1u);
-        const double2 t00 = m00, t01 = m01, t10 = m10, t11 = m11;
-        m00 = csel<double>(sw, t11, t00);
-        m01 = csel<double>(sw, t10, t01);
-        m10 = csel<double>(sw, t01, t10);
-        m11 = csel<double>(sw, t00, t11);
+        const V2<R> t00 = m00, t01 = m01, t10 = m10, t11 = m11;
+        m00 = csel<R>(sw, t11, t00);
+        m01 = csel<R>(sw, t10, t01);
+        m10 = csel<R>(sw, t01, t10);
+        m11 = csel<R>(sw, t00, t11);
     }
     // lanes whose outside controls fail, or (G < 3) whose half misses a control on group bit 3,
     // apply the identity
     const bool lane_ok = ok && (G == 3 || !(o.cm_group & 8u) || h);
     if (o.cm_ext != 0 || (G < 3 && (o.cm_group & 8u))) {
-        const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
-        m00 = csel<double>(lane_ok, m00, one);
-        m01 = csel<double>(lane_ok, m01, zero);
-        m10 = csel<double>(lane_ok, m10, zero);
-        m11 = csel<double>(lane_ok, m11, one);
+        const V2<R> one = mk<R>(1, 0), zero = mk<R>(0, 0);
+        m00 = csel<R>(lane_ok, m00, one);
+        m01 = csel<R>(lane_ok, m01, zero);
+        m10 = csel<R>(lane_ok, m10, zero);
+        m11 = csel<R>(lane_ok, m11, one);
     }
     if constexpr (G == 3) {
-        const double2 c0 = csel<double>(h, m11, m00), c1 = csel<double>(h, m10, m01);
-        hpairs<KIND, 3>(v, (int)(o.cm_group & 7u), c0, c1, c0, c1, std::make_integer_sequence<int, 8>{});
+        const V2<R> c0 = csel<R>(h, m11, m00), c1 = csel<R>(h, m10, m01);
+        hpairs<R, KIND, 3>(v, (int)(o.cm_group & 7u), c0, c1, c0, c1, std::make_integer_sequence<int, 8>{});
     } else {
-        hpairs<KIND, G>(v, (int)(o.cm_group & 7u), m00, m01, m10, m11, std::make_integer_sequence<int, 8>{});
+        hpairs<R, KIND, G>(v, (int)(o.cm_group & 7u), m00, m01, m10, m11, std::make_integer_sequence<int, 8>{});
     }
 }
 
-template <int KIND>
-__device__ __forceinline__ void hop_kind(double2 (&v)[8], const FOpDev<double>& o, bool ok, uint64_t G0, int h) {
+template <typename R, int KIND>
+__device__ __forceinline__ void hop_kind(V2<R> (&v)[8], const FOpDev<R>& o, bool ok, uint64_t G0, int h) {
     switch (o.g) {
-        case 0: hop_2x2<KIND, 0>(v, o, ok, G0, h); break;
-        case 1: hop_2x2<KIND, 1>(v, o, ok, G0, h); break;
-        case 2: hop_2x2<KIND, 2>(v, o, ok, G0, h); break;
-        default: hop_2x2<KIND, 3>(v, o, ok, G0, h); break;
+        case 0: hop_2x2<R, KIND, 0>(v, o, ok, G0, h); break;
+        case 1: hop_2x2<R, KIND, 1>(v, o, ok, G0, h); break;
+        case 2: hop_2x2<R, KIND, 2>(v, o, ok, G0, h); break;
+        default: hop_2x2<R, KIND, 3>(v, o, ok, G0, h); break;
     }
 }
 
 // X on group bit 3 with group controls cm7 (bits 0..2): registers take the partner's value where
 // the lane condition holds
-template <int... Ms>
-__device__ __forceinline__ void hswap_cross(double2 (&v)[8], int cm7, bool ok, std::integer_sequence<int, Ms...>) {
+template <typename R, int... Ms>
+__device__ __forceinline__ void hswap_cross(V2<R> (&v)[8], int cm7, bool ok, std::integer_sequence<int, Ms...>) {
     if (cm7 == 0)
-        ((v[Ms] = csel<double>(ok, xlane(v[Ms]), v[Ms])), ...);
+        ((v[Ms] = csel<R>(ok, xlane_v<R>(v[Ms]), v[Ms])), ...);
     else
-        ((((Ms & cm7) == cm7) ? (void)(v[Ms] = csel<double>(ok, xlane(v[Ms]), v[Ms])) : (void)0), ...);
+        ((((Ms & cm7) == cm7) ? (void)(v[Ms] = csel<R>(ok, xlane_v<R>(v[Ms]), v[Ms])) : (void)0), ...);
 }
 
 // relabel flush of group bit G: pairs swap where sw is set (bitwise blend)
-template <int G, int M>
-__device__ __forceinline__ void hflush_step(double2 (&v)[8], uint64_t msk) {
+template <typename R, int G, int M>
+__device__ __forceinline__ void hflush_step(V2<R> (&v)[8], UBits<R> msk) {
     if constexpr (G == 3) {
-        const double2 o = xlane(v[M]);
-        const uint64_t ax = __builtin_bit_cast(uint64_t, v[M].x), ay = __builtin_bit_cast(uint64_t, v[M].y);
-        const uint64_t bx = __builtin_bit_cast(uint64_t, o.x), by = __builtin_bit_cast(uint64_t, o.y);
-        v[M].x = __builtin_bit_cast(double, ax ^ ((ax ^ bx) & msk));
-        v[M].y = __builtin_bit_cast(double, ay ^ ((ay ^ by) & msk));
+        const V2<R> o = xlane_v<R>(v[M]);
+        const UBits<R> ax = __builtin_bit_cast(UBits<R>, v[M].x), ay = __builtin_bit_cast(UBits<R>, v[M].y);
+        const UBits<R> bx = __builtin_bit_cast(UBits<R>, o.x), by = __builtin_bit_cast(UBits<R>, o.y);
+        v[M].x = __builtin_bit_cast(R, ax ^ ((ax ^ bx) & msk));
+        v[M].y = __builtin_bit_cast(R, ay ^ ((ay ^ by) & msk));
     } else if constexpr (!((M >> G) & 1)) {
         constexpr int M1 = M | (1 << G);
-        const uint64_t ax = __builtin_bit_cast(uint64_t, v[M].x), ay = __builtin_bit_cast(uint64_t, v[M].y);
-        const uint64_t bx = __builtin_bit_cast(uint64_t, v[M1].x), by = __builtin_bit_cast(uint64_t, v[M1].y);
-        const uint64_t tx = (ax ^ bx) & msk, ty = (ay ^ by) & msk;
-        v[M].x = __builtin_bit_cast(double, ax ^ tx);
-        v[M].y = __builtin_bit_cast(double, ay ^ ty);
-        v[M1].x = __builtin_bit_cast(double, bx ^ tx);
-        v[M1].y = __builtin_bit_cast(double, by ^ ty);
+        const UBits<R> ax = __builtin_bit_cast(UBits<R>, v[M].x), ay = __builtin_bit_cast(UBits<R>, v[M].y);
+        const UBits<R> bx = __builtin_bit_cast(UBits<R>, v[M1].x), by = __builtin_bit_cast(UBits<R>, v[M1].y);
+        const UBits<R> tx = (ax ^ bx) & msk, ty = (ay ^ by) & msk;
+        v[M].x = __builtin_bit_cast(R, ax ^ tx);
+        v[M].y = __builtin_bit_cast(R, ay ^ ty);
+        v[M1].x = __builtin_bit_cast(R, bx ^ tx);
+        v[M1].y = __builtin_bit_cast(R, by ^ ty);
     }
 }
 
-template <int G, int... Ms>
-__device__ __forceinline__ void hflush(double2 (&v)[8], int sw, std::integer_sequence<int, Ms...>) {
-    const uint64_t msk = 0ull - (uint64_t)sw;
-    (hflush_step<G, Ms>(v, msk), ...);
+template <typename R, int G, int... Ms>
+__device__ __forceinline__ void hflush(V2<R> (&v)[8], int sw, std::integer_sequence<int, Ms...>) {
+    const UBits<R> msk = (UBits<R>)0 - (UBits<R>)sw;
+    (hflush_step<R, G, Ms>(v, msk), ...);
 }
 
-template <int... Ms>
-__device__ __forceinline__ void hphase(double2 (&v)[8], int cm7, double2 d, std::integer_sequence<int, Ms...>) {
+template <typename R, int... Ms>
+__device__ __forceinline__ void hphase(V2<R> (&v)[8], int cm7, V2<R> d, std::integer_sequence<int, Ms...>) {
     if (cm7 == 0)
-        ((v[Ms] = cmul<double>(d, v[Ms])), ...);
+        ((v[Ms] = cmul<R>(d, v[Ms])), ...);
     else
-        ((((Ms & cm7) == cm7) ? (void)(v[Ms] = cmul<double>(d, v[Ms])) : (void)0), ...);
+        ((((Ms & cm7) == cm7) ? (void)(v[Ms] = cmul<R>(d, v[Ms])) : (void)0), ...);
 }
 
 // factor f_S of a diagonal run applied to this lane's members (h << 3) | m that contain S
-template <int S, int... Ms>
-__device__ __forceinline__ void hrun_subset(double2 (&v)[8], const RunHead<double>* __restrict__ rh,
-                                            const RunTerm<double>* __restrict__ terms, uint32_t active,
+template <typename R, int S, int... Ms>
+__device__ __forceinline__ void hrun_subset(V2<R> (&v)[8], const RunHead<R>* __restrict__ rh,
+                                            const RunTerm<R>* __restrict__ terms, uint32_t active,
                                             uint32_t angles, uint64_t G0, int h, std::integer_sequence<int, Ms...>) {
     if (!((active >> S) & 1u)) return;
-    double2 f = mk<double>(rh->f0[2 * S], rh->f0[2 * S + 1]);
-    double th = 0;
+    V2<R> f = mk<R>(rh->f0[2 * S], rh->f0[2 * S + 1]);
+    R th = 0;
     const int t1 = rh->tbeg[S + 1];
     for (int t = rh->tbeg[S]; t < t1; t++) {
-        const RunTerm<double>& tm = terms[t];
+        const RunTerm<R>& tm = terms[t];
         if (tm.angle == 2) {
-            th += tm.v[0] * (double)(G0 & tm.cl);
+            th += tm.v[0] * (R)(G0 & tm.cl);
             continue;
         }
         const bool cond = (G0 & tm.cl) == tm.cl;
         const bool sel = tm.pm ? (bool)px_bit(G0, tm.pm, tm.pc) : (bool)(tm.pc & 1u);
         if (tm.angle) {
-            th += cond ? (sel ? tm.v[1] : tm.v[0]) : 0.0;
+            th += cond ? (sel ? tm.v[1] : tm.v[0]) : (R)0;
         } else {
-            double2 val = csel<double>(sel, mk<double>(tm.v[2], tm.v[3]), mk<double>(tm.v[0], tm.v[1]));
-            val = csel<double>(cond, val, mk<double>(1, 0));
-            f = cmul<double>(f, val);
+            V2<R> val = csel<R>(sel, mk<R>(tm.v[2], tm.v[3]), mk<R>(tm.v[0], tm.v[1]));
+            val = csel<R>(cond, val, mk<R>(1, 0));
+            f = cmul<R>(f, val);
         }
     }
     if ((angles >> S) & 1u) {
-        double sn, cs;
+        R sn, cs;
         sin_cos(th, &sn, &cs);
-        f = cmul<double>(f, mk<double>(cs, sn));
+        f = cmul<R>(f, mk<R>(cs, sn));
     }
-    if constexpr ((S & 8) != 0) f = csel<double>(h, f, mk<double>(1, 0));
+    if constexpr ((S & 8) != 0) f = csel<R>(h, f, mk<R>(1, 0));
     constexpr int S7 = S & 7;
-    ((((Ms & S7) == S7) ? (void)(v[Ms] = cmul<double>(f, v[Ms])) : (void)0), ...);
+    ((((Ms & S7) == S7) ? (void)(v[Ms] = cmul<R>(f, v[Ms])) : (void)0), ...);
 }
 
-template <int... Ss>
-__device__ __forceinline__ void hdiag_run(double2 (&v)[8], const unsigned char* __restrict__ runs,
-                                          const FOpDev<double>& o, uint64_t G0, int h, std::integer_sequence<int, Ss...>) {
-    const RunHead<double>* rh = reinterpret_cast<const RunHead<double>*>(runs + o.cm_ext);
-    const RunTerm<double>* terms = reinterpret_cast<const RunTerm<double>*>(rh + 1);
-    (hrun_subset<Ss>(v, rh, terms, (uint32_t)o.g, o.flags, G0, h, std::make_integer_sequence<int, 8>{}), ...);
+template <typename R, int... Ss>
+__device__ __forceinline__ void hdiag_run(V2<R> (&v)[8], const unsigned char* __restrict__ runs,
+                                          const FOpDev<R>& o, uint64_t G0, int h, std::integer_sequence<int, Ss...>) {
+    const RunHead<R>* rh = reinterpret_cast<const RunHead<R>*>(runs + o.cm_ext);
+    const RunTerm<R>* terms = reinterpret_cast<const RunTerm<R>*>(rh + 1);
+    (hrun_subset<R, Ss>(v, rh, terms, (uint32_t)o.g, o.flags, G0, h, std::make_integer_sequence<int, 8>{}), ...);
 }
 
-__device__ __forceinline__ void apply_hop(double2 (&v)[8], const FOpDev<double>& o, uint64_t G0, int h,
+template <typename R>
+__device__ __forceinline__ void apply_hop(V2<R> (&v)[8], const FOpDev<R>& o, uint64_t G0, int h,
                                           const unsigned char* __restrict__ runs) {
     const bool ok = (G0 & o.cm_ext) == o.cm_ext;
     switch (o.kind) {
-        case F_DENSE: hop_kind<F_DENSE>(v, o, ok, G0, h); break;
-        case F_REAL: hop_kind<F_REAL>(v, o, ok, G0, h); break;
-        case F_DIAG: hop_kind<F_DIAG>(v, o, ok, G0, h); break;
+        case F_DENSE: hop_kind<R, F_DENSE>(v, o, ok, G0, h); break;
+        case F_REAL: hop_kind<R, F_REAL>(v, o, ok, G0, h); break;
+        case F_DIAG: hop_kind<R, F_DIAG>(v, o, ok, G0, h); break;
         case F_SWAP:
             if (o.g == 3) {
-                hswap_cross(v, (int)(o.cm_group & 7u), ok, std::make_integer_sequence<int, 8>{});
+                hswap_cross<R>(v, (int)(o.cm_group & 7u), ok, std::make_integer_sequence<int, 8>{});
             } else if (o.cm_ext == 0 && !(o.cm_group & 8u)) {
-                hop_kind<F_SWAP>(v, o, ok, G0, h);  // uniform register moves
+                hop_kind<R, F_SWAP>(v, o, ok, G0, h);  // uniform register moves
             } else {
-                hop_kind<F_DENSE>(v, o, ok, G0, h);  // X as a matrix, identity where a condition fails
+                hop_kind<R, F_DENSE>(v, o, ok, G0, h);  // X as a matrix, identity where a condition fails
             }
             break;
         case F_FLUSH: {
             const int sw = px_bit(G0, o.pxmask, (o.flags >> 1) & 1u);
             switch (o.g) {
-                case 0: hflush<0>(v, sw, std::make_integer_sequence<int, 8>{}); break;
-                case 1: hflush<1>(v, sw, std::make_integer_sequence<int, 8>{}); break;
-                case 2: hflush<2>(v, sw, std::make_integer_sequence<int, 8>{}); break;
-                default: hflush<3>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                case 0: hflush<R, 0>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                case 1: hflush<R, 1>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                case 2: hflush<R, 2>(v, sw, std::make_integer_sequence<int, 8>{}); break;
+                default: hflush<R, 3>(v, sw, std::make_integer_sequence<int, 8>{}); break;
             }
             break;
         }
-        case F_DIAGRUN: hdiag_run(v, runs, o, G0, h, std::make_integer_sequence<int, 16>{}); break;
+        case F_DIAGRUN: hdiag_run<R>(v, runs, o, G0, h, std::make_integer_sequence<int, 16>{}); break;
         default: {  // F_PHASE
-            double2 d = csel<double>((G0 >> o.g) & 1ull, mk<double>(o.m[6], o.m[7]), mk<double>(o.m[0], o.m[1]));
-            d = csel<double>(ok && (!(o.cm_group & 8u) || h), d, mk<double>(1, 0));
-            hphase(v, (int)(o.cm_group & 7u), d, std::make_integer_sequence<int, 8>{});
+            V2<R> d = csel<R>((G0 >> o.g) & 1ull, mk<R>(o.m[6], o.m[7]), mk<R>(o.m[0], o.m[1]));
+            d = csel<R>(ok && (!(o.cm_group & 8u) || h), d, mk<R>(1, 0));
+            hphase<R>(v, (int)(o.cm_group & 7u), d, std::make_integer_sequence<int, 8>{});
         }
     }
 }
@@ -817,16 +827,16 @@ __device__ __forceinline__ void hrow_slots(int (&a)[8], int l0, int s0, int s1, 
     ((a[Ms] = sl ^ swz(((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2))), ...);
 }
 
-template <int... Ms>
-__device__ __forceinline__ void hgather(double2 (&v)[8], const double2* tile, const int (&a)[8],
+template <typename R, int... Ms>
+__device__ __forceinline__ void hgather(V2<R> (&v)[8], const V2<R>* tile, const int (&a)[8],
                                         std::integer_sequence<int, Ms...>) {
     ((v[Ms] = tile[a[Ms]]), ...);
 }
 
 // the slots are recomputed from the row's swizzled base (slot ^ relabel offset), so no address
 // register has to stay live across the op loop
-template <int... Ms>
-__device__ __forceinline__ void hscatter(const double2 (&v)[8], double2* tile, int sx, int s0, int s1, int s2,
+template <typename R, int... Ms>
+__device__ __forceinline__ void hscatter(const V2<R> (&v)[8], V2<R>* tile, int sx, int s0, int s1, int s2,
                                          std::integer_sequence<int, Ms...>) {
     ((tile[sx ^ swz(((Ms & 1) << s0) | (((Ms >> 1) & 1) << s1) | (((Ms >> 2) & 1) << s2))] = v[Ms]), ...);
 }
@@ -834,13 +844,13 @@ __device__ __forceinline__ void hscatter(const double2 (&v)[8], double2* tile, i
 // NS = 2: the reverse sweep, both states' tiles side by side in LDS (as k_fused_pass), gradient
 // stages evaluated from LDS between compute groups.
 // (two-state tiles fit two workgroups per CU at most, so that instantiation may use 256 registers)
-template <int NS>
-__global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __restrict__ psi, double2* __restrict__ psi1,
-                                                     const PassDesc pd, const FOpDev<double>* __restrict__ ops,
+template <typename R, int NS>
+__global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __restrict__ psi, V2<R>* __restrict__ psi1,
+                                                     const PassDesc pd, const FOpDev<R>* __restrict__ ops,
                                                      const unsigned char* __restrict__ runs,
                                                      double* __restrict__ partials, uint64_t tiles) {
     extern __shared__ unsigned char lds_raw[];
-    double2* tile = reinterpret_cast<double2*>(lds_raw);
+    V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
     const int T = 1 << pd.b;
     const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 8: two lanes per row
     const uint64_t base = tile_base(blockIdx.x, pd);
@@ -851,21 +861,27 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __
     const int L = pd.L;
     for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
     {
-        // direct HBM -> LDS loads: LDS slot lh + tid receives element lh + swz(tid)
-        const uint64_t g_sw = base | local_to_global(swz(tid), pd);
+        // element e of the tile goes to LDS slot swz(e) (the swizzle is an involution). complex128:
+        // direct HBM -> LDS loads (global_load_lds_dwordx4), slot lh + tid receiving element
+        // lh + swz(tid) (lh is a multiple of NT >= 256); complex64: 8-byte loads through registers
+        const uint64_t g_sw = base | local_to_global(sizeof(R) == 8 ? swz(tid) : tid, pd);
         const int wave = tid >> 6;
 #pragma unroll
         for (int st = 0; st < NS; st++) {
-            const double2* src_state = st ? psi1 : psi;
+            const V2<R>* src_state = st ? psi1 : psi;
 #pragma unroll 8
             for (int lh = 0; lh < T; lh += NT) {
-                const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
-                auto* dst = (__attribute__((address_space(3))) void*)(tile + st * T + lh + wave * 64);
-                __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+                if constexpr (sizeof(R) == 8) {
+                    const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
+                    auto* dst = (__attribute__((address_space(3))) void*)(tile + st * T + lh + wave * 64);
+                    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+                } else {
+                    tile[st * T + swz(lh + tid)] = src_state[g_sw | local_to_global(lh, pd)];
+                }
             }
         }
     }
-    __builtin_amdgcn_s_waitcnt(0);
+    if constexpr (sizeof(R) == 8) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     // the two lanes of a row are lane bit 3 apart, so the 8 lanes that one LDS cycle of a 16-byte
     // access serves (lanes 8k .. 8k+7) hold 8 consecutive rows, which the swizzle puts in distinct
@@ -874,7 +890,7 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __
     for (int gi = 0; gi < pd.ngroups; gi++) {
         if constexpr (NS == 2) {
             if ((pd.stage_mask >> gi) & 1u) {
-                grad_stage<double>(tile, tile + T, T, base | local_to_global(tid, pd), pd, ops + pd.gop[gi],
+                grad_stage<R>(tile, tile + T, T, base | local_to_global(tid, pd), pd, ops + pd.gop[gi],
                                    pd.gop[gi + 1] - pd.gop[gi], red, partials, tiles);
                 continue;  // grad_stage ends with a barrier
             }
@@ -884,14 +900,14 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __
         const int sl = swz(l0 | (h << s3));
         const uint64_t G0 = base | (uint64_t)(l0 & ((1 << L) - 1)) | hi_tab[l0 >> L];
         for (int st = 0; st < NS; st++) {
-            double2* tl = tile + st * T;
-            double2 v[8];
+            V2<R>* tl = tile + st * T;
+            V2<R> v[8];
             {
                 int a[8];
                 hrow_slots(a, l0 | (h << s3), s0, s1, s2, std::make_integer_sequence<int, 8>{});
-                hgather(v, tl, a, std::make_integer_sequence<int, 8>{});
+                hgather<R>(v, tl, a, std::make_integer_sequence<int, 8>{});
             }
-            for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_hop(v, ops[oi], G0, h, runs);
+            for (int oi = pd.gop[gi]; oi < pd.gop[gi + 1]; oi++) apply_hop<R>(v, ops[oi], G0, h, runs);
             // the group's relabeling; bits without a parity mask are the same for every lane
             const uint32_t pc = pd.gpxc[gi];
             int px = 0;
@@ -902,15 +918,15 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(double2* __
                 px |= bit << pd.gbits[gi][j];
             }
             // a row's two lanes are in one wavefront, whose LDS reads all precede its writes
-            hscatter(v, tl, sl ^ swz(px), s0, s1, s2, std::make_integer_sequence<int, 8>{});
+            hscatter<R>(v, tl, sl ^ swz(px), s0, s1, s2, std::make_integer_sequence<int, 8>{});
         }
         __syncthreads();
     }
     const uint64_t g_tid = base | local_to_global(tid, pd);
 #pragma unroll
     for (int st = 0; st < NS; st++) {
-        double2* dst_state = st ? psi1 : psi;
-        const double2* tl = tile + st * T;
+        V2<R>* dst_state = st ? psi1 : psi;
+        const V2<R>* tl = tile + st * T;
 #pragma unroll 8
         for (int lh = 0; lh < T; lh += NT) dst_state[g_tid | local_to_global(lh, pd)] = tl[swz(lh + tid)];
     }
@@ -1219,17 +1235,18 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
         // between passes stages its matrix there)
         double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;
-        if constexpr (sizeof(R) == 8) {
+        // split rows run complex128 passes and complex64 two-state passes (complex64 one-state
+        // passes measured no faster at 2^12 tiles and much slower at the default 2^13)
+        if constexpr (sizeof(R) == 8 || NS == 2) {
             const int hb_max = NS == 1 ? 13 : 12;  // at most 1024 / 512 threads
             if (s.split_rows && pd.b >= 11 && pd.b <= hb_max) {
                 const size_t lds_h = lds + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
                 if (lds_h > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 160 KiB of LDS");
-                if (lds_h > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_fused_half<NS>), s.device);
+                if (lds_h > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_fused_half<R, NS>), s.device);
                 {
                     KernelScope ks(s, NS == 1 ? "fused_pass" : "fused_pass2", 2.0 * NS * (double)s.size * sizeof(V2<R>));
-                    hipLaunchKernelGGL((k_fused_half<NS>), dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds_h,
-                                       s.stream, reinterpret_cast<double2*>(psi), reinterpret_cast<double2*>(psi1), pd,
-                                       reinterpret_cast<const FOpDev<double>*>(ops), druns, partials, tiles);
+                    hipLaunchKernelGGL((k_fused_half<R, NS>), dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds_h,
+                                       s.stream, psi, psi1, pd, ops, druns, partials, tiles);
                     QFVM_LAUNCHED();
                 }
                 if (cols > 0) launch_column_sums(s, partials, tiles, cols, slot_of, grads_dev);

@@ -67,6 +67,22 @@ def test_adjoint_kernel_variants(split, tile):
     assert np.abs(got - want).max() < 1e-10
 
 
+@pytest.mark.parametrize("split,tile", [(1, 12), (1, 13), (0, 12)])
+def test_adjoint_complex64_kernel_variants(split, tile):
+    """complex64 reverse sweeps on the split-row two-state kernel (2^11 / 2^12 tiles per state) and on
+    the one-lane kernel, against the oracle at the complex64 tolerance."""
+    n = 15
+    rng = np.random.Generator(np.random.PCG64(1450 + 10 * split + tile))
+    make, thetas, ops, gens = rotation_program(n, 90, rng)
+    terms, coeffs = random_pauli_terms(n, 6, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, dtype="c64",
+                            options=[("split_rows", split), ("tile_bits", tile)])
+    assert abs(e - e_want) < 1e-4
+    assert np.abs(got - want).max() < 1e-4
+
+
 def test_adjoint_complex64():
     n = 10
     rng = np.random.Generator(np.random.PCG64(1400))
