@@ -40,6 +40,8 @@ struct PassDesc {
     int gop[FUSE_MAX_GROUPS + 1];                  // op ranges in the op array
     uint64_t gpxm[FUSE_MAX_GROUPS][FUSE_GROUP_BITS];  // relabel masks at each group's end
     uint32_t gpxc[FUSE_MAX_GROUPS];                   // relabel constants at each group's end
+    uint64_t top[8];     // global offsets of local index j << (b - 3): the tile's 3 highest local bits
+    uint64_t top16[16];  // the same for j << (b - 4): the 4 highest
 };
 
 // Diagonal-run record (fused.h): 16 host factors f0, then for each subset S its term range
@@ -218,6 +220,15 @@ __device__ __forceinline__ uint64_t local_to_global(int l, const PassDesc& pd) {
     return g;
 }
 
+// the same for an index below 2^(b-3) (a thread index of the split-row kernel): the loop stops at
+// bit b - 4
+__device__ __forceinline__ uint64_t local_to_global_low(int l, const PassDesc& pd) {
+    uint64_t g = (uint64_t)(l & ((1 << pd.L) - 1));
+    for (int k = pd.L; k < pd.b - 3; k++)
+        if ((l >> k) & 1) g |= 1ull << pd.q[k];
+    return g;
+}
+
 template <typename R, int KIND>
 __device__ __forceinline__ void op_kind(V2<R> (&v)[16], const FOpDev<R>& o, bool ok, uint64_t G0) {
     switch (o.g) {
@@ -391,6 +402,9 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
     const int tid = threadIdx.x;
     const int NT = blockDim.x;  // 64 .. 512, a power of two
     const uint64_t g_tid = base | local_to_global(tid, pd);
+    // one row per thread (NT = T / 16): the loop runs over the tile's 4 highest local bits, whose
+    // global offsets come precomputed (pd.top16), so the loads issue back to back
+    const bool rows16 = NT == (T >> 4);
 #pragma unroll
     for (int st = 0; st < NS; st++) {
         const V2<R>* src_state = st ? psi1 : psi;
@@ -400,13 +414,25 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
             // receives element swz(s) = lh + swz(tid) (the swizzle only mixes bits 0-6, and lh is a
             // multiple of NT >= 128), so no VGPRs hold the tile in flight
             const uint64_t g_sw = base | local_to_global(swz(tid), pd);
-            const int wave = tid >> 6;
+            const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+            if (rows16) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const void* src = (const void*)(src_state + (g_sw | pd.top16[j]));
+                    auto* dst = (__attribute__((address_space(3))) void*)(tl + j * NT + wave * 64);
+                    if constexpr (sizeof(R) == 8) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+                }
+            } else {
 #pragma unroll 16
-            for (int lh = 0; lh < T; lh += NT) {
-                const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
-                auto* dst = (__attribute__((address_space(3))) void*)(tl + lh + wave * 64);
-                if constexpr (sizeof(R) == 8) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+                for (int lh = 0; lh < T; lh += NT) {
+                    const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
+                    auto* dst = (__attribute__((address_space(3))) void*)(tl + lh + wave * 64);
+                    if constexpr (sizeof(R) == 8) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+                }
             }
+        } else if (rows16) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) tl[swz(j * NT + tid)] = src_state[g_tid | pd.top16[j]];
         } else {
 #pragma unroll 16
             for (int lh = 0; lh < T; lh += NT)
@@ -451,9 +477,14 @@ __global__ void __launch_bounds__(512) k_fused_pass(V2<R>* __restrict__ psi, V2<
     for (int st = 0; st < NS; st++) {
         V2<R>* dst_state = st ? psi1 : psi;
         const V2<R>* tl = tile + st * T;
+        if (rows16) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) dst_state[g_tid | pd.top16[j]] = tl[swz(j * NT + tid)];
+        } else {
 #pragma unroll 4
-        for (int lh = 0; lh < T; lh += NT) {
-            if (lh + tid < T) dst_state[g_tid | local_to_global(lh, pd)] = tl[swz(lh + tid)];
+            for (int lh = 0; lh < T; lh += NT) {
+                if (lh + tid < T) dst_state[g_tid | local_to_global(lh, pd)] = tl[swz(lh + tid)];
+            }
         }
     }
 }
@@ -859,28 +890,33 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
     double* red = reinterpret_cast<double*>(tile + NS * T);
     uint64_t* hi_tab = reinterpret_cast<uint64_t*>(red + (NS == 2 ? 64 : 0));
     const int L = pd.L;
-    for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
     {
         // element e of the tile goes to LDS slot swz(e) (the swizzle is an involution). complex128:
         // direct HBM -> LDS loads (global_load_lds_dwordx4), slot lh + tid receiving element
         // lh + swz(tid) (lh is a multiple of NT >= 256); complex64: 8-byte loads through registers
-        const uint64_t g_sw = base | local_to_global(sizeof(R) == 8 ? swz(tid) : tid, pd);
-        const int wave = tid >> 6;
+        // lh = j NT = j 2^(b-3) runs over the tile's 3 highest local bits, whose global offsets come
+        // precomputed (pd.top), so the 8 loads issue back to back; the wavefront index is made
+        // uniform so the LDS destination (M0) is scalar arithmetic
+        const uint64_t g_sw = base | local_to_global_low(sizeof(R) == 8 ? swz(tid) : tid, pd);
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 #pragma unroll
         for (int st = 0; st < NS; st++) {
             const V2<R>* src_state = st ? psi1 : psi;
-#pragma unroll 8
-            for (int lh = 0; lh < T; lh += NT) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int lh = j * NT;
                 if constexpr (sizeof(R) == 8) {
-                    const void* src = (const void*)(src_state + (g_sw | local_to_global(lh, pd)));
+                    const void* src = (const void*)(src_state + (g_sw | pd.top[j]));
                     auto* dst = (__attribute__((address_space(3))) void*)(tile + st * T + lh + wave * 64);
                     __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
                 } else {
-                    tile[st * T + swz(lh + tid)] = src_state[g_sw | local_to_global(lh, pd)];
+                    tile[st * T + swz(lh + tid)] = src_state[g_sw | pd.top[j]];
                 }
             }
         }
     }
+    // the high-bit table is built while the loads are in flight
+    for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
     if constexpr (sizeof(R) == 8) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     // the two lanes of a row are lane bit 3 apart, so the 8 lanes that one LDS cycle of a 16-byte
@@ -922,13 +958,13 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
         }
         __syncthreads();
     }
-    const uint64_t g_tid = base | local_to_global(tid, pd);
+    const uint64_t g_tid = base | local_to_global_low(tid, pd);
 #pragma unroll
     for (int st = 0; st < NS; st++) {
         V2<R>* dst_state = st ? psi1 : psi;
         const V2<R>* tl = tile + st * T;
-#pragma unroll 8
-        for (int lh = 0; lh < T; lh += NT) dst_state[g_tid | local_to_global(lh, pd)] = tl[swz(lh + tid)];
+#pragma unroll
+        for (int j = 0; j < 8; j++) dst_state[g_tid | pd.top[j]] = tl[swz(j * NT + tid)];
     }
 }
 
@@ -1071,6 +1107,16 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
     pd.b = b;
     pd.L = L;
     for (int k = 0; k < b; k++) pd.q[k] = p.Q[k];
+    for (int j = 0; j < 8; j++) {
+        pd.top[j] = 0;
+        for (int i = 0; i < 3; i++)
+            if ((j >> i) & 1) pd.top[j] |= 1ull << p.Q[b - 3 + i];
+    }
+    for (int j = 0; j < 16; j++) {
+        pd.top16[j] = 0;
+        for (int i = 0; i < 4; i++)
+            if ((j >> i) & 1) pd.top16[j] |= 1ull << p.Q[b - 4 + i];
+    }
     pd.ngroups = (int)p.groups.size();
     if (pd.ngroups > FUSE_MAX_GROUPS || b > FUSE_MAX_TILE_BITS || b < FUSE_GROUP_BITS || b > s.n)
         throw Error(QFVM_ERR_RUNTIME, "fused pass exceeds kernel limits");
