@@ -265,6 +265,16 @@ def batch_leg(count: int, workers: int):
     results, wall = runner.run(items)
     runner.close()
     lat = np.array([r.latency_s for r in results]) * 1e3
+    # the same batch with every call captured into a hipGraph (option "graphs"): the first run
+    # instantiates one executable graph per circuit and handle, the timed run recaptures each call
+    # and updates its cached graph (the path a training step with new parameters takes)
+    grunner = BatchRunner(dtype="c64", workers=workers, tuner=tuner, graphs=True)
+    grunner.warm([it.n for it in items])
+    grunner.run(items)
+    gresults, gwall = grunner.run(items)
+    grunner.close()
+    glat = np.array([r.latency_s for r in gresults]) * 1e3
+    gdiff = max(float(np.abs(a.grads - b.grads).max(initial=0.0)) for a, b in zip(gresults, results))
     return {
         "workload": f"{count} circuits, n 10-22, depth 4-16 (RY+RZ layers, CZ ring), c64, <sum Z> + adjoint gradient",
         "circuits_per_s": round(count / wall, 1),
@@ -276,6 +286,13 @@ def batch_leg(count: int, workers: int):
         "host_compile_s": round(t_compile, 2),
         "tune_and_warm_s": round(t_warm, 2),
         "tiles": {str(k[0]): list(v) for k, v in sorted(tuner.choice.items())},
+        "graphs": {
+            "circuits_per_s": round(count / gwall, 1),
+            "wall_ms": round(gwall * 1e3, 2),
+            "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(glat, 99)), 3),
+            "max_abs_diff_vs_eager": gdiff,
+        },
     }
 
 

@@ -101,10 +101,17 @@ class Autotuner:
 
 @dataclass
 class BatchRunner:
+    """`graphs=True` runs every engine call as a captured hipGraph (option "graphs"): each worker
+    slot owns its handles and a fixed share of the items (longest-first onto the least-loaded
+    slot), so a circuit evaluated again (a training loop with new parameters) lands on the handle
+    that already holds its executable graph and replays it."""
+
     dtype: str = "c64"
     workers: int = 8
     tuner: Optional[Autotuner] = None
+    graphs: bool = False
     _local: threading.local = field(default_factory=threading.local, repr=False)
+    _slots: dict = field(default_factory=dict, repr=False)
     _handles: list = field(default_factory=list, repr=False)
     _lock: threading.Lock = field(default_factory=threading.Lock, repr=False)
     _pool: Optional[ThreadPoolExecutor] = field(default=None, repr=False)
@@ -112,10 +119,14 @@ class BatchRunner:
     def __post_init__(self):
         self._pool = ThreadPoolExecutor(self.workers)
 
-    def _device(self, n: int) -> _native.DeviceState:
-        states = getattr(self._local, "states", None)
-        if states is None:
-            states = self._local.states = {}
+    def _device(self, n: int, slot: Optional[int] = None) -> _native.DeviceState:
+        if slot is None:
+            states = getattr(self._local, "states", None)
+            if states is None:
+                states = self._local.states = {}
+        else:
+            with self._lock:
+                states = self._slots.setdefault(slot, {})
         dev = states.get(n)
         if dev is None:
             dev = _native.DeviceState(n, self.dtype)
@@ -123,18 +134,32 @@ class BatchRunner:
                 tb, lb = self.tuner.tune(n, self.dtype)
                 dev.set_option("tile_bits", tb)
                 dev.set_option("low_bits", lb)
+            if self.graphs:
+                dev.set_option("graphs", 1)
             states[n] = dev
             with self._lock:
                 self._handles.append(dev)
         return dev
 
-    def _run_one(self, item: BatchItem, t0: float) -> BatchResult:
+    def _run_one(self, item: BatchItem, t0: float, slot: Optional[int] = None) -> BatchResult:
         start = time.perf_counter()
-        dev = self._device(item.n)
+        dev = self._device(item.n, slot)
         dev.init_zero()
         grads, e = dev.adjoint_prepared(item.call)
         end = time.perf_counter()
         return BatchResult(e, grads, end - start, start - t0)
+
+    def _partition(self, items: Sequence[BatchItem]) -> List[List[int]]:
+        """Item indices per worker slot: longest first (cost ~ 2^n x ops) onto the least-loaded
+        slot; the same items give the same partition."""
+        cost = [(1 << it.n) * (int(it.call.cp.n_ops) + 8) for it in items]
+        load = [0] * self.workers
+        parts: List[List[int]] = [[] for _ in range(self.workers)]
+        for k in sorted(range(len(items)), key=lambda k: (-cost[k], k)):
+            w = min(range(self.workers), key=lambda w: (load[w], w))
+            parts[w].append(k)
+            load[w] += cost[k]
+        return parts
 
     def warm(self, ns: Sequence[int]) -> None:
         """Tune, then create every worker's handles for these qubit counts (outside any timing)."""
@@ -144,15 +169,26 @@ class BatchRunner:
                 self.tuner.tune(n, self.dtype)
         barrier = threading.Barrier(self.workers)
 
-        def per_worker(_):
+        def per_worker(w):
             barrier.wait()  # one task per worker thread
             for n in ns:
-                self._device(n)
+                self._device(n, w if self.graphs else None)
 
         list(self._pool.map(per_worker, range(self.workers)))
 
     def run(self, items: Sequence[BatchItem]) -> Tuple[List[BatchResult], float]:
         """Evaluates every item; returns the results in input order and the wall time."""
+        if self.graphs:
+            parts = self._partition(items)
+            results: List[Optional[BatchResult]] = [None] * len(items)
+            t0 = time.perf_counter()
+
+            def per_slot(w):
+                for k in parts[w]:
+                    results[k] = self._run_one(items[k], t0, w)
+
+            list(self._pool.map(per_slot, range(self.workers)))
+            return results, time.perf_counter() - t0
         t0 = time.perf_counter()
         results = list(self._pool.map(lambda it: self._run_one(it, t0), items))
         return results, time.perf_counter() - t0

@@ -393,6 +393,7 @@ struct Runner {
 // ------------------------------------------------------------------------------------------------
 void* State::ensure_scratch(size_t bytes) {
     if (bytes > scratch_bytes) {
+        before_alloc();
         if (scratch) {
             QFVM_HIP(hipStreamSynchronize(stream));
             QFVM_HIP(hipFree(scratch));
@@ -407,6 +408,7 @@ void* State::ensure_scratch(size_t bytes) {
 
 double* State::ensure_pinned(size_t bytes) {
     if (bytes > host_pinned_bytes) {
+        before_alloc();
         if (host_pinned) {
             QFVM_HIP(hipStreamSynchronize(stream));
             QFVM_HIP(hipHostFree(host_pinned));
@@ -421,6 +423,7 @@ double* State::ensure_pinned(size_t bytes) {
 
 void* State::ensure_ops(size_t bytes) {
     if (bytes > ops_bytes) {
+        before_alloc();
         if (ops_buf) {
             QFVM_HIP(hipStreamSynchronize(stream));
             QFVM_HIP(hipFree(ops_buf));
@@ -436,6 +439,7 @@ void* State::ensure_ops(size_t bytes) {
 
 void* State::ensure_aux(size_t bytes) {
     if (bytes > aux_bytes) {
+        before_alloc();
         if (aux_buf) {
             QFVM_HIP(hipStreamSynchronize(stream));
             QFVM_HIP(hipFree(aux_buf));
@@ -451,8 +455,10 @@ void* State::ensure_aux(size_t bytes) {
 
 unsigned char* State::stage_host(int k, size_t bytes) {
     Stage& st = stages[k];
-    if (st.done) QFVM_HIP(hipEventSynchronize(st.done));
+    // (a capture follows a synchronised call, so every earlier copy out of the slot has finished)
+    if (st.done && !capturing) QFVM_HIP(hipEventSynchronize(st.done));
     if (bytes > st.host_bytes) {
+        before_alloc();
         if (st.host) QFVM_HIP(hipHostFree(st.host));
         st.host = nullptr;
         st.host_bytes = 0;
@@ -466,6 +472,7 @@ unsigned char* State::stage_host(int k, size_t bytes) {
 unsigned char* State::stage_dev(int k, size_t bytes) {
     Stage& st = stages[k];
     if (bytes > st.dev_bytes) {
+        before_alloc();
         if (st.dev) {
             QFVM_HIP(hipStreamSynchronize(stream));  // queued kernels may still read the old area
             QFVM_HIP(hipFree(st.dev));
@@ -481,14 +488,29 @@ unsigned char* State::stage_dev(int k, size_t bytes) {
 
 void State::stage_upload(int k, size_t bytes) {
     Stage& st = stages[k];
-    if (!st.done) QFVM_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
     QFVM_HIP(hipMemcpyAsync(st.dev, st.host, bytes, hipMemcpyHostToDevice, stream));
+    if (capturing) return;  // the graph's copy node reads the host area when the graph runs
+    if (!st.done) QFVM_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
     QFVM_HIP(hipEventRecord(st.done, stream));
 }
 
 void* State::ensure_psi2() {
-    if (!psi2) QFVM_HIP(hipMalloc(&psi2, size * amp_bytes()));
+    if (!psi2) {
+        before_alloc();
+        QFVM_HIP(hipMalloc(&psi2, size * amp_bytes()));
+    }
     return psi2;
+}
+
+void State::before_alloc() {
+    if (capturing) throw NeedsAlloc{};
+    buf_gen++;  // cached graphs name the old buffers
+}
+
+void State::drop_graphs() {
+    for (GraphEntry& e : graph_cache)
+        if (e.exec) hipGraphExecDestroy(e.exec);
+    graph_cache.clear();
 }
 
 static void destroy_state(State* s) {
@@ -496,6 +518,7 @@ static void destroy_state(State* s) {
     hipSetDevice(s->device);
     if (s->stream) hipStreamSynchronize(s->stream);
     s->resolve_events();
+    s->drop_graphs();
     if (s->psi) hipFree(s->psi);
     if (s->psi2) hipFree(s->psi2);
     for (Stage& st : s->stages) {
@@ -510,6 +533,134 @@ static void destroy_state(State* s) {
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
+
+// ------------------------------------------------------------------------------------------------
+// Graph mode. A qfvm_adjoint call queues 20 to 200 launches plus three uploads and one read-back;
+// with option "graphs" the call captures that work on its stream (relaxed capture: the host plans
+// and fills the pinned upload slots exactly as in an eager call) and launches it as one executable
+// graph. Executable graphs are cached per program structure (opcodes, qubits, differentiated ops,
+// Pauli strings; not angles, matrices or coefficients): a later call of the same structure updates
+// the cached graph's node parameters (hipGraphExecUpdate) instead of instantiating a new one, so a
+// training loop over fixed circuits with new parameters replays graphs. Captured graphs hold kernel
+// and copy nodes only (zero fills are kernels). Work that would allocate during the capture
+// abandons it and runs eagerly, which grows the buffer; a (re)allocation drops the handle's cached
+// graphs, which name the old buffers.
+
+constexpr size_t GRAPH_CACHE_MAX = 64;
+
+static uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t k = 0; k < n; k++) h = (h ^ b[k]) * 1099511628211ull;
+    return h;
+}
+
+// structure of one qfvm_adjoint call (everything that shapes the plan except parameter values)
+static uint64_t adjoint_key(const State& s, const qfvm_program* prog, const int64_t* grad_ops, const char* grad_gen,
+                            int64_t n_grads, const char* chars, const int32_t* pos, const int64_t* off,
+                            const int32_t* len, int64_t n_terms) {
+    uint64_t h = 1469598103934665603ull;
+    const int32_t opts[] = {s.n, s.dtype, s.fusion, s.tile_bits, s.low_bits, s.max_groups, s.pass_cost, s.split_rows,
+                            s.stream_tiles, s.stagger, s.stagger_shift};
+    h = fnv(h, opts, sizeof(opts));
+    h = fnv(h, &prog->n_ops, sizeof(prog->n_ops));
+    for (int64_t k = 0; k < prog->n_ops; k++) {
+        const qfvm_op& o = prog->ops[k];
+        const int32_t f[4] = {o.code, o.n_ctrl, o.n_qubits, o.n_cbits};
+        h = fnv(h, f, sizeof(f));
+        if (o.n_qubits > 0) h = fnv(h, prog->idx + o.q_off, (size_t)o.n_qubits * sizeof(int32_t));
+    }
+    h = fnv(h, &n_grads, sizeof(n_grads));
+    if (n_grads > 0) {
+        h = fnv(h, grad_ops, (size_t)n_grads * sizeof(int64_t));
+        h = fnv(h, grad_gen, (size_t)n_grads);
+    }
+    h = fnv(h, &n_terms, sizeof(n_terms));
+    for (int64_t t = 0; t < n_terms; t++) {
+        h = fnv(h, &len[t], sizeof(int32_t));
+        if (len[t] > 0) {
+            h = fnv(h, chars + off[t], (size_t)len[t]);
+            h = fnv(h, pos + off[t], (size_t)len[t] * sizeof(int32_t));
+        }
+    }
+    return h;
+}
+
+// dense multi-target gates stage their matrix from pageable host memory: such calls run eagerly
+static bool graph_safe(const std::vector<Gate>& gates) {
+    for (const Gate& g : gates)
+        if (g.kind == G_MULTI) return false;
+    return true;
+}
+
+// Captures `enqueue` on the state's stream and launches it as a graph. Returns false, with
+// nothing queued, when the work needs an allocation first.
+template <typename F>
+static bool run_graph(State& s, uint64_t key, F&& enqueue) {
+    if (s.graph_gen != s.buf_gen) {  // buffers were reallocated since the cached graphs were made
+        s.drop_graphs();
+        s.graph_gen = s.buf_gen;
+    }
+    hipGraph_t graph = nullptr;
+    QFVM_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
+    s.capturing = true;
+    try {
+        enqueue();
+    } catch (...) {
+        s.capturing = false;
+        hipGraph_t junk = nullptr;
+        const hipError_t e = hipStreamEndCapture(s.stream, &junk);
+        if (e == hipSuccess && junk) hipGraphDestroy(junk);
+        (void)hipGetLastError();
+        try {
+            throw;
+        } catch (const NeedsAlloc&) {
+            return false;
+        }
+    }
+    s.capturing = false;
+    QFVM_HIP(hipStreamEndCapture(s.stream, &graph));
+    struct GraphGuard {
+        hipGraph_t g;
+        ~GraphGuard() {
+            if (g) hipGraphDestroy(g);
+        }
+    } guard{graph};
+    State::GraphEntry* entry = nullptr;
+    for (State::GraphEntry& e : s.graph_cache)
+        if (e.key == key) entry = &e;
+    bool ready = false;
+    if (entry && entry->exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(entry->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            ready = true;
+            s.kstats[(size_t)s.stat_index("graph_update")].second.launches++;
+        } else {
+            (void)hipGetLastError();
+            hipGraphExecDestroy(entry->exec);
+            entry->exec = nullptr;
+        }
+    } else if (!entry) {
+        if (s.graph_cache.size() >= GRAPH_CACHE_MAX) {  // evict the least recently used graph
+            size_t lru = 0;
+            for (size_t k = 1; k < s.graph_cache.size(); k++)
+                if (s.graph_cache[k].last_use < s.graph_cache[lru].last_use) lru = k;
+            if (s.graph_cache[lru].exec) hipGraphExecDestroy(s.graph_cache[lru].exec);
+            s.graph_cache.erase(s.graph_cache.begin() + (std::ptrdiff_t)lru);
+        }
+        s.graph_cache.push_back(State::GraphEntry{key, 0, nullptr});
+        entry = &s.graph_cache.back();
+    }
+    if (!ready) {
+        QFVM_HIP(hipGraphInstantiate(&entry->exec, graph, nullptr, nullptr, 0));
+        s.kstats[(size_t)s.stat_index("graph_instantiate")].second.launches++;
+    }
+    entry->last_use = ++s.graph_clock;
+    QFVM_HIP(hipGraphLaunch(entry->exec, s.stream));
+    s.kstats[(size_t)s.stat_index("graph_launch")].second.launches++;
+    return true;
+}
+
 
 extern "C" {
 
@@ -704,13 +855,7 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
         // forward passes run, and waits once at the end. Results: grads[0 .. n), <H> at [n].
         const size_t rbytes = ((size_t)n_grads + 1) * sizeof(double);
         double* rdev = reinterpret_cast<double*>(s->ensure_aux(rbytes));
-        QFVM_HIP(hipMemsetAsync(rdev, 0, rbytes, s->stream));
-        apply_gates(*s, fwd);
-        void* lam = s->ensure_psi2();
-        if (H.groups.empty())
-            QFVM_HIP(hipMemsetAsync(lam, 0, s->size * s->amp_bytes(), s->stream));
-        else
-            apply_hamiltonian(*s, H, lam, rdev + n_grads);
+        double* hbuf = reinterpret_cast<double*>(s->stage_host(STAGE_RESULT, rbytes));
         // reverse sweep (gradiant.py:160-172): the marker of op k sits before U_k^dagger, where both
         // states are taken just after U_k
         std::vector<Gate> rev;
@@ -719,9 +864,22 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
             const auto r = range[(size_t)k];
             for (size_t j = r.second; j > r.first; j--) rev.push_back(dagger(fwd[j - 1]));
         }
-        if (n_grads > 0) apply_gates2(*s, rev, lam, rdev);
-        double* hbuf = reinterpret_cast<double*>(s->stage_host(STAGE_RESULT, rbytes));
-        QFVM_HIP(hipMemcpyAsync(hbuf, rdev, rbytes, hipMemcpyDeviceToHost, s->stream));
+        auto enqueue = [&] {
+            launch_zero(*s, rdev, rbytes);
+            apply_gates(*s, fwd);
+            void* lam = s->ensure_psi2();
+            if (H.groups.empty())
+                launch_zero(*s, lam, s->size * s->amp_bytes());
+            else
+                apply_hamiltonian(*s, H, lam, rdev + n_grads);
+            if (n_grads > 0) apply_gates2(*s, rev, lam, rdev);
+            QFVM_HIP(hipMemcpyAsync(hbuf, rdev, rbytes, hipMemcpyDeviceToHost, s->stream));
+        };
+        bool queued = false;
+        if (s->graphs && !s->profiling && graph_safe(fwd))
+            queued = run_graph(*s, adjoint_key(*s, prog, grad_ops, grad_gen, n_grads, chars, pos, off, len, n_terms),
+                               enqueue);
+        if (!queued) enqueue();
         QFVM_HIP(hipStreamSynchronize(s->stream));
         if (n_grads > 0) std::memcpy(grads, hbuf, (size_t)n_grads * sizeof(double));
         if (expectation) *expectation = hbuf[n_grads];
@@ -819,6 +977,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
         } else if (k == "stagger") {
             if (value < 0 || value > 64) throw Error(QFVM_ERR_VALUE, "stagger must be in 0..64");
             s->stagger = (int)value;
+        } else if (k == "graphs") {
+            s->graphs = value != 0;
         } else if (k == "stagger_shift") {
             if (value < 0 || value > 16) throw Error(QFVM_ERR_VALUE, "stagger_shift must be in 0..16");
             s->stagger_shift = (int)value;

@@ -145,6 +145,19 @@ struct State {
     int split_rows = 1;     // fused-pass kernel with two lanes per 16-amplitude row (complex128)
     int stagger = 0;        // start delay of half the persistent workgroups, in 8128-cycle sleeps
     int stagger_shift = 0;  // which blockIdx bit picks the delayed half
+    // graph mode (option "graphs"): qfvm_adjoint captures its stream work into a hipGraph and
+    // replays it through an executable graph cached per program structure (see capi.cpp)
+    int graphs = 0;
+    bool capturing = false;  // the stream is being captured: allocations and host waits throw NeedsAlloc
+    uint64_t buf_gen = 0;    // bumped by every (re)allocation of a buffer that queued work may name
+    uint64_t graph_gen = 0;  // buf_gen the cached graphs were captured under
+    uint64_t graph_clock = 0;
+    struct GraphEntry {
+        uint64_t key;
+        uint64_t last_use;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graph_cache;
     // counters: always on; event timing only when `profiling`
     bool profiling = false;
     std::vector<std::pair<std::string, KStat>> kstats;
@@ -165,7 +178,13 @@ struct State {
     void* ensure_psi2();
     int stat_index(const char* name);
     void resolve_events();  // fold finished event pairs into kstats (call after a stream sync)
+    void before_alloc();    // called before any buffer (re)allocation
+    void drop_graphs();
 };
+
+// Thrown when queued work would (re)allocate a buffer while the stream is being captured: the
+// capture is abandoned and the call runs eagerly, which grows the buffer for later captures.
+struct NeedsAlloc {};
 
 // Brackets one kernel launch: counts it and, in profiling mode, records HIP events around it on
 // the state's stream.
@@ -204,6 +223,7 @@ struct Hamiltonian {
 // kernel launchers (defined in the .hip files)
 void launch_gate(State& s, const Gate& g);
 void launch_fill_basis(State& s, uint64_t index);
+void launch_zero(State& s, void* p, size_t bytes);  // bytes: a multiple of 8
 void launch_h2d_convert(State& s, const double* host, uint64_t n);  // host c128 -> device
 void launch_d2h_convert(State& s, double* host, uint64_t n);
 double reduce_norm2(State& s);

@@ -602,6 +602,17 @@ __global__ void __launch_bounds__(512) k_fused_stream(double2* __restrict__ psi,
 // per-lane coefficients (m00, m01 where h = 0; m11, m10 where h = 1). Controls on group bit 3 and
 // the diagonal-run subsets that contain it become per-lane conditions.
 
+// streaming store: the state is far larger than L2 and the MALL, and no later kernel of the pass
+// sequence reads a line before it is evicted, so stores carry the non-temporal hint
+template <typename R>
+__device__ __forceinline__ void store_nt(V2<R>* p, V2<R> v) {
+    typedef R N2 __attribute__((ext_vector_type(2)));
+    N2 n;
+    n.x = v.x;
+    n.y = v.y;
+    __builtin_nontemporal_store(n, reinterpret_cast<N2*>(p));
+}
+
 // the partner lane's (lane ^ 8) copy of a value: DPP row_ror:8 (0x128) rotates each 16-lane row by
 // 8; every lane of the wavefront must be active
 __device__ __forceinline__ double xlane(double x) {
@@ -908,7 +919,7 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
                 if constexpr (sizeof(R) == 8) {
                     const void* src = (const void*)(src_state + (g_sw | pd.top[j]));
                     auto* dst = (__attribute__((address_space(3))) void*)(tile + st * T + lh + wave * 64);
-                    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
                 } else {
                     tile[st * T + swz(lh + tid)] = src_state[g_sw | pd.top[j]];
                 }
@@ -964,7 +975,7 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
         V2<R>* dst_state = st ? psi1 : psi;
         const V2<R>* tl = tile + st * T;
 #pragma unroll
-        for (int j = 0; j < 8; j++) dst_state[g_tid | pd.top[j]] = tl[swz(j * NT + tid)];
+        for (int j = 0; j < 8; j++) store_nt<R>(&dst_state[g_tid | pd.top[j]], tl[swz(j * NT + tid)]);
     }
 }
 

@@ -138,6 +138,13 @@ __global__ void k_fill_basis_c128(double2* psi, uint64_t size, uint64_t index) {
         psi[i] = make_double2(i == index ? 1.0 : 0.0, 0.0);
 }
 
+// zero fill of 8-byte words (a kernel rather than hipMemsetAsync, so that captured graphs hold
+// only kernel and copy nodes)
+__global__ void k_zero_words(uint64_t* p, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
 __global__ void k_fill_basis_c64(float2* psi, uint64_t size, uint64_t index) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride)
@@ -286,6 +293,15 @@ void launch_gate(State& s, const Gate& g) {
         gate1_typed<double>(s, g);
     else
         gate1_typed<float>(s, g);
+}
+
+void launch_zero(State& s, void* p, size_t bytes) {
+    if (bytes % 8 != 0) throw Error(QFVM_ERR_RUNTIME, "zero fill of a partial word");
+    const uint64_t n = bytes / 8;
+    if (n == 0) return;
+    const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(n, 256), 8192);
+    hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, s.stream, reinterpret_cast<uint64_t*>(p), n);
+    QFVM_LAUNCHED();
 }
 
 void launch_fill_basis(State& s, uint64_t index) {
