@@ -243,7 +243,7 @@ def qft_leg(n: int, shots: int, samples: int = 100000):
     return out
 
 
-def batch_leg(count: int, workers: int):
+def batch_leg(count: int, workers: int, graphs: bool = False):
     """Batched uneven workload: `count` seeded circuits (PCG64 seed 20250004), n ~ U{10..22}, depth ~
     U{4..16} layers of RY and RZ on every qubit plus a CZ ring, complex64; each evaluates <sum_i Z_i>
     and its full adjoint gradient. `workers` host threads drive one engine handle (one HIP stream)
@@ -265,16 +265,25 @@ def batch_leg(count: int, workers: int):
     results, wall = runner.run(items)
     runner.close()
     lat = np.array([r.latency_s for r in results]) * 1e3
-    # the same batch with every call captured into a hipGraph (option "graphs"): the first run
-    # instantiates one executable graph per circuit and handle, the timed run recaptures each call
-    # and updates its cached graph (the path a training step with new parameters takes)
-    grunner = BatchRunner(dtype="c64", workers=workers, tuner=tuner, graphs=True)
-    grunner.warm([it.n for it in items])
-    grunner.run(items)
-    gresults, gwall = grunner.run(items)
-    grunner.close()
-    glat = np.array([r.latency_s for r in gresults]) * 1e3
-    gdiff = max(float(np.abs(a.grads - b.grads).max(initial=0.0)) for a, b in zip(gresults, results))
+    graph_mode = None
+    if graphs:
+        # the same batch with every call captured into a hipGraph (option "graphs"): the first run
+        # instantiates one executable graph per circuit and handle, the timed run recaptures each
+        # call and updates its cached graph (the path a training step with new parameters takes)
+        grunner = BatchRunner(dtype="c64", workers=workers, tuner=tuner, graphs=True)
+        grunner.warm([it.n for it in items])
+        grunner.run(items)
+        gresults, gwall = grunner.run(items)
+        grunner.close()
+        glat = np.array([r.latency_s for r in gresults]) * 1e3
+        graph_mode = {
+            "circuits_per_s": round(count / gwall, 1),
+            "wall_ms": round(gwall * 1e3, 2),
+            "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(glat, 99)), 3),
+            "max_abs_diff_vs_eager": max(float(np.abs(a.grads - b.grads).max(initial=0.0))
+                                         for a, b in zip(gresults, results)),
+        }
     return {
         "workload": f"{count} circuits, n 10-22, depth 4-16 (RY+RZ layers, CZ ring), c64, <sum Z> + adjoint gradient",
         "circuits_per_s": round(count / wall, 1),
@@ -286,13 +295,7 @@ def batch_leg(count: int, workers: int):
         "host_compile_s": round(t_compile, 2),
         "tune_and_warm_s": round(t_warm, 2),
         "tiles": {str(k[0]): list(v) for k, v in sorted(tuner.choice.items())},
-        "graphs": {
-            "circuits_per_s": round(count / gwall, 1),
-            "wall_ms": round(gwall * 1e3, 2),
-            "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
-            "latency_ms_p99": round(float(np.percentile(glat, 99)), 3),
-            "max_abs_diff_vs_eager": gdiff,
-        },
+        "graphs": graph_mode,
     }
 
 
@@ -319,6 +322,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-circuit leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--graph-batch", action="store_true",
+                    help="also run the batch with every engine call as a captured hipGraph (not under a profiler)")
     ap.add_argument("--qft-n", type=int, default=33)
     ap.add_argument("--qft-shots", type=int, default=1000000)
     args = ap.parse_args(argv)
@@ -418,7 +423,7 @@ def main(argv=None) -> int:
         out["adjoint"] = None
     if not args.no_batch:
         print(f"[bench] batch leg: {args.batch_count} circuits", file=sys.stderr, flush=True)
-        out["batch"] = batch_leg(args.batch_count, args.batch_workers)
+        out["batch"] = batch_leg(args.batch_count, args.batch_workers, args.graph_batch)
     else:
         out["batch"] = None
     if not args.no_qft:

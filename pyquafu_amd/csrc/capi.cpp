@@ -600,6 +600,10 @@ static bool run_graph(State& s, uint64_t key, F&& enqueue) {
         s.drop_graphs();
         s.graph_gen = s.buf_gen;
     }
+    // an eager call may still be copying out of an upload slot; the capture rewrites the slots'
+    // host areas at once, so those copies must have finished
+    for (Stage& st : s.stages)
+        if (st.done) QFVM_HIP(hipEventSynchronize(st.done));
     hipGraph_t graph = nullptr;
     QFVM_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     s.capturing = true;
