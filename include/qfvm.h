@@ -257,6 +257,12 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 0 = one workgroup per tile
  *   "stagger"     persistent kernel: start delay of half the workgroups, in 8128-cycle sleeps (0..64)
  *   "stagger_shift" the blockIdx bit that selects the delayed half (0..16)
+ *   "graphs"      1 = qfvm_adjoint captures its stream work (uploads from pinned slots, kernels,
+ *                 the result read-back) into a hipGraph and launches it; executable graphs are
+ *                 cached per program structure (up to 64 per handle) and updated in place by later
+ *                 calls of the same structure with new angles, matrices or coefficients. Calls with
+ *                 dense multi-target gates, calls that must grow a device buffer, and profiling mode
+ *                 run eagerly. 0 = (default) eager launches. Results are identical either way.
  * Results do not depend on the options beyond floating-point rounding.
  */
 int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
