@@ -243,9 +243,9 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
 /*
  * Execution options of one handle (no counterpart in the reference). Keys:
  *   "fusion"      1 = plan gates into fused LDS-tiled passes (default), 0 = one kernel per gate
- *   "tile_bits"   qubits per fused tile b (4..14; 0 = default: 11 for complex128, 13 for complex64;
+ *   "tile_bits"   qubits per fused tile b (4..14; 0 = default: 11 for complex128, 12 for complex64;
  *                 the adjoint's two-state sweep uses b - 1 of an explicit value, 11 / 12 by default)
- *   "low_bits"    low qubits always in a tile L (0 = default: 5 for complex128 one-state passes, else 6)
+ *   "low_bits"    low qubits always in a tile L (0 = default: 5 for one-state passes, 6 for two-state)
  *   "max_groups"  4-qubit register groups per fused pass (1..24, default 12)
  *   "pass_cost"   compute budget of one fused pass in quarter dense-gate units (0 = default)
  *   "split_rows"  1 = (default) complex128 one-state passes of 2^11..2^13-amplitude tiles and the

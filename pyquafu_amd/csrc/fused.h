@@ -90,10 +90,11 @@ struct FuseConfig {
 
 // Default tile shape per precision and state count (states = 2: the adjoint's two-state sweep, both
 // tiles in LDS). complex128: 2^11-amplitude tiles, which let four workgroups share a CU (faster than
-// 2^12 at 24 to 33 qubits by 1.5 to 3 %), with 32-amplitude contiguous runs for one state and 64 for
-// two; complex64: 2^13 (2^12 for two states) with 64-amplitude runs.
-inline int default_tile_bits(int dtype, int states) { return dtype == QFVM_C128 ? 11 : 13 - (states - 1); }
-inline int default_low_bits(int dtype, int states) { return dtype == QFVM_C128 && states == 1 ? 5 : 6; }
+// 2^12 at 24 to 33 qubits by 1.5 to 3 %); complex64: 2^12 (one state: 41% of HBM peak per 30-qubit
+// pass against 39% at 2^13, 33-qubit QFT 234 vs 249 ms). Contiguous runs of 32 amplitudes for one
+// state and 64 for two.
+inline int default_tile_bits(int dtype, int /*states*/) { return dtype == QFVM_C128 ? 11 : 12; }
+inline int default_low_bits(int /*dtype*/, int states) { return states == 1 ? 5 : 6; }
 
 // One item of an execution plan: a fused pass or a single gate for the per-gate kernels.
 struct PlanItem {
