@@ -248,8 +248,8 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *   "low_bits"    low qubits always in a tile L (0 = default: 5 for one-state passes, 6 for two-state)
  *   "max_groups"  4-qubit register groups per fused pass (1..24, default 12)
  *   "pass_cost"   compute budget of one fused pass in quarter dense-gate units (0 = default)
- *   "split_rows"  1 = (default) complex128 one-state passes of 2^11..2^13-amplitude tiles and the
- *                 two-state passes of the adjoint (both precisions, 2^11 or 2^12 per state) run
+ *   "split_rows"  1 = (default) complex128 one-state passes of 2^10..2^13-amplitude tiles and the
+ *                 two-state passes of the adjoint (both precisions, 2^10 to 2^12 per state) run
  *                 with two lanes per 16-amplitude register row (twice the wavefronts per tile),
  *                 0 = one lane per row
  *   "stream"      1 = persistent fused-pass kernel that prefetches the next tile (complex128,

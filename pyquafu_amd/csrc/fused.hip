@@ -911,7 +911,7 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
     {
         // element e of the tile goes to LDS slot swz(e) (the swizzle is an involution). complex128:
         // direct HBM -> LDS loads (global_load_lds_dwordx4), slot lh + tid receiving element
-        // lh + swz(tid) (lh is a multiple of NT >= 256); complex64: 8-byte loads through registers
+        // lh + swz(tid) (lh is a multiple of NT >= 128); complex64: 8-byte loads through registers
         // lh = j NT = j 2^(b-3) runs over the tile's 3 highest local bits, whose global offsets come
         // precomputed (pd.top), so the 8 loads issue back to back; the wavefront index is made
         // uniform so the LDS destination (M0) is scalar arithmetic
@@ -931,7 +931,7 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
                     auto* dst = (__attribute__((address_space(3))) void*)(tile + st * T + lh + wave * 64);
                     __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
                 } else {
-                    tile[st * T + swz(lh + tid)] = src_state[g_sw | pd.top[j]];
+                    tile[st * T + swz(lh + tid)] = load_nt<R>(&src_state[g_sw | pd.top[j]]);
                 }
             }
         }
@@ -1305,10 +1305,10 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         // between passes stages its matrix there)
         double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;
         // split rows run complex128 passes and complex64 two-state passes (complex64 one-state
-        // passes measured no faster at 2^12 tiles and much slower at the default 2^13)
+        // passes measured slower on them: 30-qubit circuit at 2^12 tiles 822 vs 744 ms)
         if constexpr (sizeof(R) == 8 || NS == 2) {
             const int hb_max = NS == 1 ? 13 : 12;  // at most 1024 / 512 threads
-            if (s.split_rows && pd.b >= 11 && pd.b <= hb_max) {
+            if (s.split_rows && pd.b >= 10 && pd.b <= hb_max) {
                 const size_t lds_h = lds + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
                 if (lds_h > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "split-row tile exceeds 160 KiB of LDS");
                 if (lds_h > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_fused_half<R, NS>), s.device);

@@ -52,7 +52,7 @@ def test_adjoint_tile_shapes(tile, low, groups):
     assert np.abs(got - want).max() < 1e-10
 
 
-@pytest.mark.parametrize("split,tile", [(1, 12), (1, 13), (0, 12), (0, 13)])
+@pytest.mark.parametrize("split,tile", [(1, 11), (1, 12), (1, 13), (0, 12), (0, 13)])
 def test_adjoint_kernel_variants(split, tile):
     """The reverse sweep on the split-row two-state kernel (two lanes per row; option tile_bits = b + 1
     for two-state passes) and on the one-lane kernel, against the oracle."""
@@ -67,9 +67,9 @@ def test_adjoint_kernel_variants(split, tile):
     assert np.abs(got - want).max() < 1e-10
 
 
-@pytest.mark.parametrize("split,tile", [(1, 12), (1, 13), (0, 12)])
+@pytest.mark.parametrize("split,tile", [(1, 11), (1, 12), (1, 13), (0, 12)])
 def test_adjoint_complex64_kernel_variants(split, tile):
-    """complex64 reverse sweeps on the split-row two-state kernel (2^11 / 2^12 tiles per state) and on
+    """complex64 reverse sweeps on the split-row two-state kernel (2^10 .. 2^12 tiles per state) and on
     the one-lane kernel, against the oracle at the complex64 tolerance."""
     n = 15
     rng = np.random.Generator(np.random.PCG64(1450 + 10 * split + tile))

@@ -289,7 +289,7 @@ def test_errors_are_reported():
 
 
 # ------------------------------------------------------------------ fused passes
-# kernel variants of complex128 one-state passes: split rows (the default for 2^11 / 2^12 tiles),
+# kernel variants of complex128 one-state passes: split rows (the default for 2^10 .. 2^13 tiles),
 # one lane per row, and the persistent prefetching kernel
 KERNELS = {"split": (("split_rows", 1),), "classic": (("split_rows", 0),), "stream": (("split_rows", 0), ("stream", 1))}
 
@@ -298,7 +298,9 @@ KERNELS = {"split": (("split_rows", 1),), "classic": (("split_rows", 0),), "stre
     ("c128", 12, 6, 12, "split"), ("c128", 12, 6, 12, "classic"), ("c128", 12, 6, 12, "stream"),
     ("c128", 11, 5, 12, "split"), ("c128", 11, 3, 6, "stream"), ("c128", 12, 3, 4, "split"),
     ("c128", 12, 3, 4, "classic"), ("c128", 13, 4, 24, "stream"), ("c128", 13, 4, 24, "classic"),
-    ("c128", 8, 2, 2, "split"), ("c64", 13, 6, 12, "split"), ("c64", 14, 4, 24, "split")])
+    ("c128", 8, 2, 2, "split"), ("c64", 13, 6, 12, "split"), ("c64", 14, 4, 24, "split"),
+    ("c128", 10, 5, 12, "split"), ("c128", 10, 6, 3, "split"), ("c128", 10, 3, 24, "split"),
+    ("c64", 12, 5, 12, "split")])
 def test_fused_matches_oracle(dtype, tile, low, groups, kernel):
     rng = np.random.Generator(np.random.PCG64(tile * 100 + low))
     n = 17
