@@ -138,6 +138,16 @@ def adjoint_leg(n: int, layers: int, fd_params: int = 8):
     adjoint_once()
     prof = dev.profile(reset=True)
     dev.set_profiling(False)
+    # the same gradient with 4 forward-state checkpoints restored during the reverse sweep
+    dev.set_option("checkpoints", 4)
+    adjoint_once()
+    ck_times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ck_grads, _ = adjoint_once()
+        ck_times.append(time.perf_counter() - t0)
+    dev.set_option("checkpoints", 0)
+    t_ck = float(np.median(ck_times))
 
     def energy(p):
         dev.init_zero()
@@ -178,6 +188,8 @@ def adjoint_leg(n: int, layers: int, fd_params: int = 8):
         "max_abs_diff_vs_param_shift": float(np.abs(grads - ps).max()),
         "max_abs_diff_vs_central_fd": float(np.abs(grads[:fd_params] - fd).max()),
         "expectation": e,
+        "checkpoints": {"count": 4, "adjoint_ms": round(t_ck * 1e3, 3),
+                        "max_abs_diff_vs_plain": float(np.abs(ck_grads - grads).max())},
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in prof.items()},
     }
 

@@ -257,6 +257,10 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 0 = one workgroup per tile
  *   "stagger"     persistent kernel: start delay of half the workgroups, in 8128-cycle sleeps (0..64)
  *   "stagger_shift" the blockIdx bit that selects the delayed half (0..16)
+ *   "checkpoints" K (0..64, default 0): qfvm_adjoint keeps up to K copies of the forward state at
+ *                 op boundaries spread over the differentiated range and restores psi from them in
+ *                 the reverse sweep instead of uncomputing through them; K shrinks to the copies that
+ *                 free device memory holds (a reserve of 1/32 of the device memory is kept)
  *   "graphs"      1 = qfvm_adjoint captures its stream work (uploads from pinned slots, kernels,
  *                 the result read-back) into a hipGraph and launches it; executable graphs are
  *                 cached per program structure (up to 64 per handle) and updated in place by later

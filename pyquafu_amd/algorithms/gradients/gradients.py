@@ -69,15 +69,18 @@ def adjoint_program(qc):
 
 
 def grad_adjoint(qc, hamiltonian, psi_in=np.array([], dtype=complex), *, precision: str = "c128",
-                 return_expectation: bool = False):
+                 return_expectation: bool = False, checkpoints: int = 0):
     """Reverse-mode gradient (arXiv:2009.02823) of <H> with respect to qc.variables' order of
-    `_calc_parameter_grads`. With return_expectation, returns (grads, <H>)."""
+    `_calc_parameter_grads`. With return_expectation, returns (grads, <H>). `checkpoints` > 0 keeps
+    that many copies of the forward state on the device (fewer when device memory is short) and
+    restores them during the reverse sweep instead of uncomputing through them."""
     if not qc._parameter_grads:  # pylint: disable=protected-access
         qc.get_parameter_grads()
     para_grads = qc._calc_parameter_grads()  # pylint: disable=protected-access
     prog, ops, gens, diff = adjoint_program(qc)
     terms, coeffs = hamiltonian_terms(hamiltonian)
     dev = _call(_native.pool.get, qc.num, precision)
+    _call(dev.set_option, "checkpoints", int(checkpoints))
     psi_in = np.asarray(psi_in)
     if psi_in.size:
         _call(dev.load, psi_in)

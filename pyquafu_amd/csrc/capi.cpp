@@ -455,8 +455,19 @@ void* State::ensure_aux(size_t bytes) {
 
 unsigned char* State::stage_host(int k, size_t bytes) {
     Stage& st = stages[k];
-    // (a capture follows a synchronised call, so every earlier copy out of the slot has finished)
-    if (st.done && !capturing) QFVM_HIP(hipEventSynchronize(st.done));
+    if (capturing) {
+        // a captured copy reads its host area when the graph runs, so every upload of one capture
+        // gets its own stretch of the capture arena (one slot may upload several times per call)
+        const size_t off = (cap_off + 255) / 256 * 256;
+        if (off + bytes > cap_host_bytes) {
+            cap_need = std::max({cap_need, 2 * (off + bytes), (size_t)1 << 20});
+            throw NeedsAlloc{};
+        }
+        cap_off = off + bytes;
+        st.cur_host = cap_host + off;
+        return st.cur_host;
+    }
+    if (st.done) QFVM_HIP(hipEventSynchronize(st.done));
     if (bytes > st.host_bytes) {
         before_alloc();
         if (st.host) QFVM_HIP(hipHostFree(st.host));
@@ -466,6 +477,7 @@ unsigned char* State::stage_host(int k, size_t bytes) {
         QFVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.host), want, hipHostMallocDefault));
         st.host_bytes = want;
     }
+    st.cur_host = st.host;
     return st.host;
 }
 
@@ -488,8 +500,8 @@ unsigned char* State::stage_dev(int k, size_t bytes) {
 
 void State::stage_upload(int k, size_t bytes) {
     Stage& st = stages[k];
-    QFVM_HIP(hipMemcpyAsync(st.dev, st.host, bytes, hipMemcpyHostToDevice, stream));
-    if (capturing) return;  // the graph's copy node reads the host area when the graph runs
+    QFVM_HIP(hipMemcpyAsync(st.dev, st.cur_host, bytes, hipMemcpyHostToDevice, stream));
+    if (capturing) return;  // the graph's copy node reads the arena when the graph runs
     if (!st.done) QFVM_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
     QFVM_HIP(hipEventRecord(st.done, stream));
 }
@@ -500,6 +512,15 @@ void* State::ensure_psi2() {
         QFVM_HIP(hipMalloc(&psi2, size * amp_bytes()));
     }
     return psi2;
+}
+
+void* State::ensure_checkpoint(size_t k) {
+    while (ckpt.size() <= k) ckpt.push_back(nullptr);
+    if (!ckpt[k]) {
+        before_alloc();
+        QFVM_HIP(hipMalloc(&ckpt[k], size * amp_bytes()));
+    }
+    return ckpt[k];
 }
 
 void State::before_alloc() {
@@ -519,8 +540,11 @@ static void destroy_state(State* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     s->resolve_events();
     s->drop_graphs();
+    if (s->cap_host) hipHostFree(s->cap_host);
     if (s->psi) hipFree(s->psi);
     if (s->psi2) hipFree(s->psi2);
+    for (void* c : s->ckpt)
+        if (c) hipFree(c);
     for (Stage& st : s->stages) {
         if (st.host) hipHostFree(st.host);
         if (st.dev) hipFree(st.dev);
@@ -600,10 +624,21 @@ static bool run_graph(State& s, uint64_t key, F&& enqueue) {
         s.drop_graphs();
         s.graph_gen = s.buf_gen;
     }
-    // an eager call may still be copying out of an upload slot; the capture rewrites the slots'
-    // host areas at once, so those copies must have finished
+    // an eager call may still be copying out of an upload slot, which the capture does not wait for
     for (Stage& st : s.stages)
         if (st.done) QFVM_HIP(hipEventSynchronize(st.done));
+    s.cap_need = std::max(s.cap_need, (size_t)1 << 20);
+    if (s.cap_need > s.cap_host_bytes) {  // first capture, or an earlier one ran out of arena
+        s.before_alloc();
+        s.drop_graphs();  // their copy nodes read the old arena
+        s.graph_gen = s.buf_gen;
+        if (s.cap_host) QFVM_HIP(hipHostFree(s.cap_host));
+        s.cap_host = nullptr;
+        s.cap_host_bytes = 0;
+        QFVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.cap_host), s.cap_need, hipHostMallocDefault));
+        s.cap_host_bytes = s.cap_need;
+    }
+    s.cap_off = 0;
     hipGraph_t graph = nullptr;
     QFVM_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     s.capturing = true;
@@ -860,29 +895,70 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
         const size_t rbytes = ((size_t)n_grads + 1) * sizeof(double);
         double* rdev = reinterpret_cast<double*>(s->ensure_aux(rbytes));
         double* hbuf = reinterpret_cast<double*>(s->stage_host(STAGE_RESULT, rbytes));
+        // Checkpoints: K op boundaries b_0 < .. < b_{K-1} spread evenly over the differentiated range
+        // [begin, n_ops); the forward sweep copies psi at each (the state before op b_j) and the
+        // reverse sweep, having uncomputed op b_j, restores psi from that copy, so rounding from
+        // U^dagger does not accumulate across segments. K shrinks to what free device memory holds
+        // beyond the copies already allocated (keeping a reserve of 1/32 of the device memory).
+        const size_t sbytes = s->size * s->amp_bytes();
+        int64_t K = n_grads > 0 ? std::min<int64_t>(s->checkpoints, prog->n_ops - begin - 1) : 0;
+        if (K > (int64_t)s->ckpt.size()) {
+            size_t free_b = 0, total_b = 0;
+            QFVM_HIP(hipMemGetInfo(&free_b, &total_b));
+            const size_t reserve = total_b / 32;
+            const int64_t fit = free_b > reserve ? (int64_t)((free_b - reserve) / sbytes) : 0;
+            K = std::min<int64_t>(K, (int64_t)s->ckpt.size() + fit);
+        }
+        K = std::max<int64_t>(K, 0);
+        std::vector<int64_t> bound((size_t)K);
+        for (int64_t j = 0; j < K; j++) bound[(size_t)j] = begin + (j + 1) * (prog->n_ops - begin) / (K + 1);
         // reverse sweep (gradiant.py:160-172): the marker of op k sits before U_k^dagger, where both
-        // states are taken just after U_k
-        std::vector<Gate> rev;
-        for (int64_t k = prog->n_ops - 1; k >= begin; k--) {
-            if (marker_of[(size_t)k] >= 0) rev.push_back(markers[(size_t)marker_of[(size_t)k]]);
-            const auto r = range[(size_t)k];
-            for (size_t j = r.second; j > r.first; j--) rev.push_back(dagger(fwd[j - 1]));
+        // states are taken just after U_k; one gate list per segment between checkpoints
+        std::vector<std::vector<Gate>> rev((size_t)K + 1);
+        {
+            int64_t seg = K;
+            for (int64_t k = prog->n_ops - 1; k >= begin; k--) {
+                if (marker_of[(size_t)k] >= 0) rev[(size_t)seg].push_back(markers[(size_t)marker_of[(size_t)k]]);
+                const auto r = range[(size_t)k];
+                for (size_t j = r.second; j > r.first; j--) rev[(size_t)seg].push_back(dagger(fwd[j - 1]));
+                if (seg > 0 && k == bound[(size_t)seg - 1]) seg--;
+            }
         }
         auto enqueue = [&] {
             launch_zero(*s, rdev, rbytes);
-            apply_gates(*s, fwd);
+            size_t g0 = 0;
+            for (int64_t j = 0; j < K; j++) {
+                const size_t g1 = range[(size_t)bound[(size_t)j]].first;
+                apply_gates(*s, std::vector<Gate>(fwd.begin() + (std::ptrdiff_t)g0, fwd.begin() + (std::ptrdiff_t)g1));
+                void* c = s->ensure_checkpoint((size_t)j);
+                QFVM_HIP(hipMemcpyAsync(c, s->psi, sbytes, hipMemcpyDeviceToDevice, s->stream));
+                s->kstats[(size_t)s->stat_index("checkpoint_copy")].second.launches++;
+                g0 = g1;
+            }
+            apply_gates(*s, K > 0 ? std::vector<Gate>(fwd.begin() + (std::ptrdiff_t)g0, fwd.end()) : fwd);
             void* lam = s->ensure_psi2();
             if (H.groups.empty())
-                launch_zero(*s, lam, s->size * s->amp_bytes());
+                launch_zero(*s, lam, sbytes);
             else
                 apply_hamiltonian(*s, H, lam, rdev + n_grads);
-            if (n_grads > 0) apply_gates2(*s, rev, lam, rdev);
+            if (n_grads > 0) {
+                for (int64_t seg = K; seg >= 0; seg--) {
+                    apply_gates2(*s, rev[(size_t)seg], lam, rdev);
+                    if (seg > 0) {
+                        QFVM_HIP(hipMemcpyAsync(s->psi, s->ckpt[(size_t)seg - 1], sbytes, hipMemcpyDeviceToDevice,
+                                                s->stream));
+                        s->kstats[(size_t)s->stat_index("checkpoint_restore")].second.launches++;
+                    }
+                }
+            }
             QFVM_HIP(hipMemcpyAsync(hbuf, rdev, rbytes, hipMemcpyDeviceToHost, s->stream));
         };
         bool queued = false;
-        if (s->graphs && !s->profiling && graph_safe(fwd))
-            queued = run_graph(*s, adjoint_key(*s, prog, grad_ops, grad_gen, n_grads, chars, pos, off, len, n_terms),
-                               enqueue);
+        if (s->graphs && !s->profiling && graph_safe(fwd)) {
+            uint64_t key = adjoint_key(*s, prog, grad_ops, grad_gen, n_grads, chars, pos, off, len, n_terms);
+            key = fnv(key, &K, sizeof(K));
+            queued = run_graph(*s, key, enqueue);
+        }
         if (!queued) enqueue();
         QFVM_HIP(hipStreamSynchronize(s->stream));
         if (n_grads > 0) std::memcpy(grads, hbuf, (size_t)n_grads * sizeof(double));
@@ -983,6 +1059,9 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->stagger = (int)value;
         } else if (k == "graphs") {
             s->graphs = value != 0;
+        } else if (k == "checkpoints") {
+            if (value < 0 || value > 64) throw Error(QFVM_ERR_VALUE, "checkpoints must be in 0..64");
+            s->checkpoints = (int)value;
         } else if (k == "stagger_shift") {
             if (value < 0 || value > 16) throw Error(QFVM_ERR_VALUE, "stagger_shift must be in 0..16");
             s->stagger_shift = (int)value;

@@ -114,6 +114,7 @@ struct Stage {
     unsigned char* dev = nullptr;
     size_t dev_bytes = 0;
     hipEvent_t done = nullptr;
+    unsigned char* cur_host = nullptr;  // host area of the latest stage_host (`host`, or the capture arena)
 };
 
 // per-handle state
@@ -145,6 +146,10 @@ struct State {
     int split_rows = 1;     // fused-pass kernel with two lanes per 16-amplitude row (complex128)
     int stagger = 0;        // start delay of half the persistent workgroups, in 8128-cycle sleeps
     int stagger_shift = 0;  // which blockIdx bit picks the delayed half
+    // adjoint checkpoints (option "checkpoints"): up to this many copies of the forward state,
+    // restored into psi during the reverse sweep instead of continuing the uncompute past them
+    int checkpoints = 0;
+    std::vector<void*> ckpt;
     // graph mode (option "graphs"): qfvm_adjoint captures its stream work into a hipGraph and
     // replays it through an executable graph cached per program structure (see capi.cpp)
     int graphs = 0;
@@ -152,6 +157,10 @@ struct State {
     uint64_t buf_gen = 0;    // bumped by every (re)allocation of a buffer that queued work may name
     uint64_t graph_gen = 0;  // buf_gen the cached graphs were captured under
     uint64_t graph_clock = 0;
+    unsigned char* cap_host = nullptr;  // pinned capture arena: the host areas of captured uploads
+    size_t cap_host_bytes = 0;
+    size_t cap_off = 0;
+    size_t cap_need = 0;  // arena size an abandoned capture asked for
     struct GraphEntry {
         uint64_t key;
         uint64_t last_use;
@@ -176,6 +185,7 @@ struct State {
     unsigned char* stage_dev(int slot, size_t bytes);
     void stage_upload(int slot, size_t bytes);           // host area -> device area, async
     void* ensure_psi2();
+    void* ensure_checkpoint(size_t k);  // state-sized buffer k (allocated on first use)
     int stat_index(const char* name);
     void resolve_events();  // fold finished event pairs into kstats (call after a stream sync)
     void before_alloc();    // called before any buffer (re)allocation

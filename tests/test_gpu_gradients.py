@@ -288,3 +288,48 @@ def test_batch_runner_matches_single_calls():
         want, e_want = og.adjoint(it.prog, it.n, it.ops, it.gens, it.terms, it.coeffs)
         assert abs(r.expectation - e_want) < 1e-11 and abs(e - e_want) < 1e-11
         assert np.abs(r.grads - want).max() < 1e-10 and np.abs(g - want).max() < 1e-10
+
+
+@pytest.mark.parametrize("dtype,K,n", [("c128", 1, 12), ("c128", 3, 12), ("c128", 7, 9), ("c64", 4, 13)])
+def test_adjoint_checkpoints(dtype, K, n):
+    """Option "checkpoints": the forward sweep keeps K copies of psi and the reverse sweep restores
+    psi from them instead of continuing the uncompute. Gradients against the oracle and against the
+    run without checkpoints; the handle ends in the state before the first differentiated op."""
+    rng = np.random.Generator(np.random.PCG64(1500 + 10 * K + n))
+    make, thetas, ops, gens = rotation_program(n, 160, rng)
+    terms, coeffs = random_pauli_terms(n, 5, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    tol = 1e-10 if dtype == "c128" else 1e-4
+    plain = _native.DeviceState(n, dtype)
+    ck = _native.DeviceState(n, dtype)
+    try:
+        ck.set_option("checkpoints", K)
+        plain.init_zero()
+        g0, e0 = plain.adjoint(prog, ops, gens, terms, coeffs)
+        for _ in range(2):  # the second call reuses the checkpoint buffers
+            ck.init_zero()
+            g1, e1 = ck.adjoint(prog, ops, gens, terms, coeffs)
+            assert abs(e1 - e_want) < tol
+            assert np.abs(g1 - want).max() < tol
+            assert np.abs(g1 - g0).max() < tol
+        p = ck.profile()
+        assert p["checkpoint_copy"]["launches"] == 2 * K
+        assert p["checkpoint_restore"]["launches"] == 2 * K
+        assert np.abs(ck.store() - plain.store()).max() < (1e-12 if dtype == "c128" else 1e-5)
+    finally:
+        plain.close()
+        ck.close()
+
+
+def test_adjoint_checkpoints_more_than_ops():
+    """More checkpoints than op boundaries: one per boundary at most."""
+    n = 8
+    rng = np.random.Generator(np.random.PCG64(1590))
+    make, thetas, ops, gens = rotation_program(n, 6, rng)
+    terms, coeffs = random_pauli_terms(n, 3, rng)
+    prog = make(thetas)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, options=[("checkpoints", 64)])
+    assert abs(e - e_want) < 1e-11
+    assert np.abs(got - want).max() < 1e-10

@@ -171,3 +171,24 @@ def test_batch_runner_graph_steps_match_eager():
     finally:
         eager.close()
         graph.close()
+
+
+def test_graph_replays_with_checkpoints():
+    n = 12
+    eager = _native.DeviceState(n, "c128")
+    graph = _native.DeviceState(n, "c128")
+    try:
+        for d in (eager, graph):
+            d.set_option("checkpoints", 3)
+        graph.set_option("graphs", 1)
+        ham = tfim(n)
+        for rep in range(4):
+            args = call_args(layered(n, 4, 900 + rep), ham)
+            g0, e0 = run(eager, args)
+            g1, e1 = run(graph, args)
+            assert np.array_equal(g1, g0)
+            assert e1 == e0
+        assert graph_counts(graph)["graph_update"] >= 2
+    finally:
+        eager.close()
+        graph.close()
