@@ -583,7 +583,7 @@ static uint64_t adjoint_key(const State& s, const qfvm_program* prog, const int6
                             int64_t n_grads, const char* chars, const int32_t* pos, const int64_t* off,
                             const int32_t* len, int64_t n_terms) {
     uint64_t h = 1469598103934665603ull;
-    const int32_t opts[] = {s.n, s.dtype, s.fusion, s.tile_bits, s.low_bits, s.max_groups, s.pass_cost, s.split_rows,
+    const int32_t opts[] = {s.n, s.dtype, s.mfma, s.fusion, s.tile_bits, s.low_bits, s.max_groups, s.pass_cost, s.split_rows,
                             s.stream_tiles, s.stagger, s.stagger_shift};
     h = fnv(h, opts, sizeof(opts));
     h = fnv(h, &prog->n_ops, sizeof(prog->n_ops));
@@ -1059,6 +1059,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->stagger = (int)value;
         } else if (k == "graphs") {
             s->graphs = value != 0;
+        } else if (k == "mfma") {
+            s->mfma = value != 0;
         } else if (k == "checkpoints") {
             if (value < 0 || value > 64) throw Error(QFVM_ERR_VALUE, "checkpoints must be in 0..64");
             s->checkpoints = (int)value;

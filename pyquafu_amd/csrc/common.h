@@ -149,6 +149,7 @@ struct State {
     // adjoint checkpoints (option "checkpoints"): up to this many copies of the forward state,
     // restored into psi during the reverse sweep instead of continuing the uncompute past them
     int checkpoints = 0;
+    int mfma = 1;  // complex128 dense gates on 5 or 6 targets run on the matrix cores (option "mfma")
     std::vector<void*> ckpt;
     // graph mode (option "graphs"): qfvm_adjoint captures its stream work into a hipGraph and
     // replays it through an executable graph cached per program structure (see capi.cpp)
@@ -234,6 +235,8 @@ struct Hamiltonian {
 void launch_gate(State& s, const Gate& g);
 void launch_fill_basis(State& s, uint64_t index);
 void launch_zero(State& s, void* p, size_t bytes);  // bytes: a multiple of 8
+// raises a kernel's dynamic LDS limit to 160 KiB, once per (kernel, device); thread-safe
+void allow_large_lds(const void* fn, int device);
 void launch_h2d_convert(State& s, const double* host, uint64_t n);  // host c128 -> device
 void launch_d2h_convert(State& s, double* host, uint64_t n);
 double reduce_norm2(State& s);

@@ -1207,7 +1207,7 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
 
 // Raises a kernel's dynamic LDS limit to 160 KiB, once per (kernel, device); handles run on several
 // host threads.
-static void allow_large_lds(const void* fn, int device) {
+void allow_large_lds(const void* fn, int device) {
     static std::mutex lock;
     static std::vector<std::pair<const void*, int>> done;
     std::lock_guard<std::mutex> guard(lock);

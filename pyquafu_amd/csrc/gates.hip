@@ -96,6 +96,77 @@ __global__ void __launch_bounds__(256) k_multi_reg(V2<R>* __restrict__ psi, uint
     }
 }
 
+// Dense 2^K x 2^K complex128 matrix on the FP64 matrix cores (K = 5, 6). The gate is a real GEMM:
+// with X the tile's amplitudes as columns (rows 0..D-1 real parts, D..2D-1 imaginary parts) and
+// A = [[Ur, -Ui], [Ui, Ur]], Y = A X. One workgroup (4 wavefronts) takes NG = 64 groups: gather
+// into LDS, each wavefront computes 16 columns with v_mfma_f64_16x16x4f64 (A streamed from L2, X
+// from LDS), results back through LDS, scatter. At K >= 5 a dense gate moves 2 x 16 B per amplitude
+// but costs 16 x 2^K flops per amplitude (>= 16 flop/B, past the FP64 ridge of ~9.8 flop/B), so it
+// is compute-bound and the matrix cores carry the arithmetic.
+// Operand layouts of v_mfma_f64_16x16x4f64 (one 16x16 output block per wavefront):
+//   A (16 x 4): lane l supplies A[l % 16][l / 16];  B (4 x 16): lane l supplies B[l / 16][l % 16];
+//   D (16 x 16): lane l holds D[4 v + l / 16][l % 16] in accumulator element v = 0..3
+//   (measured on gfx950 with one-hot operands).
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int MFMA_NG = 64;               // groups per workgroup tile
+constexpr int MFMA_PITCH = MFMA_NG + 16;  // LDS row pitch in doubles (rows k and k + 1 in other banks)
+
+template <int K>
+__global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
+                                                    uint64_t cmask, TMask<K> tm, const double* __restrict__ at,
+                                                    int member_fast) {
+    constexpr int D = 1 << K, M = 2 * D, P = MFMA_PITCH, RT = M / 16;
+    extern __shared__ double xs[];  // M x P
+    const uint64_t g0 = (uint64_t)blockIdx.x * MFMA_NG;
+    const int tid = threadIdx.x;
+    // element e of the tile: member m, column c; member-fast order keeps consecutive lanes on
+    // consecutive addresses when a target is among the lowest qubits, column-fast otherwise
+    auto where = [&](int e, int& m, int& c) {
+        if (member_fast) {
+            m = e & (D - 1);
+            c = e >> K;
+        } else {
+            c = e & (MFMA_NG - 1);
+            m = e / MFMA_NG;
+        }
+    };
+    for (int e = tid; e < D * MFMA_NG; e += 256) {
+        int m, c;
+        where(e, m, c);
+        const uint64_t g = g0 + (uint64_t)c;
+        double2 v = make_double2(0.0, 0.0);
+        if (g < ngroups) v = psi[insert_zeros(g, sp) | cmask | tm.m[m]];
+        xs[m * P + c] = v.x;
+        xs[(m + D) * P + c] = v.y;
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const int col = wave * 16 + (lane & 15), kq = lane >> 4;
+    dx4 acc[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) acc[rt] = dx4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int k0 = 0; k0 < M; k0 += 4) {
+        const double b = xs[(k0 + kq) * P + col];
+        const double* arow = at + (size_t)(k0 + kq) * M + (lane & 15);  // at[k][r] = A[r][k]
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+            acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[rt * 16], b, acc[rt], 0, 0, 0);
+    }
+    __syncthreads();  // every wavefront has read its columns of X
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) xs[(rt * 16 + 4 * v + kq) * P + col] = acc[rt][v];
+    __syncthreads();
+    for (int e = tid; e < D * MFMA_NG; e += 256) {
+        int m, c;
+        where(e, m, c);
+        const uint64_t g = g0 + (uint64_t)c;
+        if (g < ngroups) psi[insert_zeros(g, sp) | cmask | tm.m[m]] = make_double2(xs[m * P + c], xs[(m + D) * P + c]);
+    }
+}
+
 // Dense k-target matrix, K >= 6: one block per group; the group is staged in LDS.
 struct TPos {
     int k;
@@ -246,6 +317,45 @@ static void multi_reg_launch(State& s, const Gate& g, const InsertSpec& sp, uint
     QFVM_LAUNCHED();
 }
 
+template <int K>
+static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uint64_t ngroups) {
+    constexpr int D = 1 << K, M = 2 * D;
+    TMask<K> tm{};
+    for (int m = 0; m < D; m++) {
+        uint64_t o = 0;
+        for (int b = 0; b < K; b++)
+            if ((m >> b) & 1) o |= 1ull << g.targets[b];
+        tm.m[m] = o;
+    }
+    // A^T of the real embedding, at[k * M + r] = A[r][k], staged through scratch
+    std::vector<double> at((size_t)M * M);
+    for (int r = 0; r < D; r++)
+        for (int q = 0; q < D; q++) {
+            const cd u = g.mk[(size_t)r * D + q];
+            at[(size_t)q * M + r] = u.real();
+            at[(size_t)(q + D) * M + r] = -u.imag();
+            at[(size_t)q * M + r + D] = u.imag();
+            at[(size_t)(q + D) * M + r + D] = u.real();
+        }
+    double* dat = reinterpret_cast<double*>(s.ensure_scratch(at.size() * sizeof(double)));
+    QFVM_HIP(hipMemcpyAsync(dat, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    int lowest = 64;
+    for (int t : g.targets) lowest = std::min(lowest, t);
+    int lowest_free = 0;  // lowest qubit outside the gate
+    while (lowest_free < s.n && (((g.cmask >> lowest_free) & 1) || std::find(g.targets.begin(), g.targets.end(), lowest_free) != g.targets.end()))
+        lowest_free++;
+    const int member_fast = lowest < lowest_free ? 1 : 0;
+    const size_t lds = (size_t)M * MFMA_PITCH * sizeof(double);
+    if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_multi_mfma<K>), s.device);
+    const uint64_t blocks = (ngroups + MFMA_NG - 1) / MFMA_NG;
+    if (blocks > 0xffffffffull) throw Error(QFVM_ERR_VALUE, "dense gate grid too large");
+    KernelScope ks(s, "multi_mfma", (double)ngroups * D * 2.0 * sizeof(double2));
+    hipLaunchKernelGGL((k_multi_mfma<K>), dim3((unsigned)blocks), dim3(256), lds, s.stream,
+                       reinterpret_cast<double2*>(s.psi), ngroups, sp, g.cmask, tm, dat, member_fast);
+    QFVM_LAUNCHED();
+    QFVM_HIP(hipStreamSynchronize(s.stream));  // the host matrix copy must outlive the async copy
+}
+
 template <typename R>
 static void multi_typed(State& s, const Gate& g) {
     const int k = (int)g.targets.size();
@@ -256,6 +366,15 @@ static void multi_typed(State& s, const Gate& g) {
     pos.insert(pos.end(), g.targets.begin(), g.targets.end());
     const InsertSpec sp = make_spec(pos);
     const uint64_t ngroups = s.size >> pos.size();
+    if constexpr (sizeof(R) == 8) {
+        if (s.mfma && (k == 5 || k == 6)) {
+            if (k == 5)
+                multi_mfma_launch<5>(s, g, sp, ngroups);
+            else
+                multi_mfma_launch<6>(s, g, sp, ngroups);
+            return;
+        }
+    }
     // matrix -> device (scratch), converted to the state's precision
     std::vector<V2<R>> hm(dim * dim);
     for (uint64_t q = 0; q < dim * dim; q++) hm[q] = mk<R>((R)g.mk[q].real(), (R)g.mk[q].imag());

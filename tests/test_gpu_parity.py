@@ -499,3 +499,45 @@ def test_full_size_round_trip_30_qubits():
         assert abs(dev.norm2() - 1.0) < 1e-11
     finally:
         dev.close()
+
+
+# ------------------------------------------------------------------ dense gates on the matrix cores
+@pytest.mark.parametrize("k,where,nc", [(5, "low", 0), (5, "high", 1), (5, "mixed", 2), (6, "low", 1),
+                                        (6, "high", 0), (6, "mixed", 2)])
+def test_dense_mfma_matches_oracle(k, where, nc):
+    """complex128 dense gates on 5 and 6 targets run on v_mfma_f64_16x16x4f64 (option mfma, the
+    default); against the oracle and against the VALU kernels (mfma = 0), with the targets among
+    the lowest qubits (member-fast tile order), the highest, or spread, and with controls."""
+    from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
+    from tests.progs import random_unitary
+
+    n = 15
+    rng = np.random.Generator(np.random.PCG64(1700 + 10 * k + nc + (0 if where == "low" else 3 if where == "high" else 6)))
+    if where == "low":
+        targets = [int(x) for x in rng.permutation(k)]
+    elif where == "high":
+        targets = [int(x) for x in rng.permutation(np.arange(n - k, n))]
+    else:
+        targets = [int(x) for x in rng.choice(n, k, replace=False)]
+    rest = [q for q in range(n) if q not in targets]
+    ctrls = [int(x) for x in rng.choice(rest, nc, replace=False)]
+    b = ProgramBuilder()
+    for _ in range(2):
+        b.add(MULTI, ctrls + targets, n_ctrl=nc, matrix=random_unitary(1 << k, rng))
+    prog = b.program(n_qubits=n)
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    cpu.apply(prog)
+    want = cpu.store()
+    got = {}
+    for mfma in (1, 0):
+        dev = gpu_state(n)
+        dev.set_option("mfma", mfma)
+        dev.load(psi0)
+        dev.apply(prog)
+        got[mfma] = dev.store()
+        prof = dev.profile()
+        assert ("multi_mfma" in prof) == bool(mfma)
+    assert np.abs(got[1] - want).max() < TOL["c128"]
+    assert np.abs(got[1] - got[0]).max() < TOL["c128"]
