@@ -99,8 +99,8 @@ __global__ void __launch_bounds__(256) k_multi_reg(V2<R>* __restrict__ psi, uint
 // Dense 2^K x 2^K complex128 matrix on the FP64 matrix cores (K = 5, 6). The gate is a real GEMM:
 // with X the tile's amplitudes as columns (rows 0..D-1 real parts, D..2D-1 imaginary parts) and
 // A = [[Ur, -Ui], [Ui, Ur]], Y = A X. One workgroup (4 wavefronts) takes NG = 64 groups: gather
-// into LDS, each wavefront computes 16 columns with v_mfma_f64_16x16x4f64 (A streamed from L2, X
-// from LDS), results back through LDS, scatter. At K >= 5 a dense gate moves 2 x 16 B per amplitude
+// into LDS, the wavefronts multiply with v_mfma_f64_16x16x4f64 (A held in registers, X from LDS),
+// results back through LDS, scatter. At K >= 5 a dense gate moves 2 x 16 B per amplitude
 // but costs 16 x 2^K flops per amplitude (>= 16 flop/B, past the FP64 ridge of ~9.8 flop/B), so it
 // is compute-bound and the matrix cores carry the arithmetic.
 // Operand layouts of v_mfma_f64_16x16x4f64 (one 16x16 output block per wavefront):
@@ -111,15 +111,25 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int MFMA_NG = 64;               // groups per workgroup tile
 constexpr int MFMA_PITCH = MFMA_NG + 16;  // LDS row pitch in doubles (rows k and k + 1 in other banks)
 
+// Persistent: each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ... Wavefront w owns the
+// row blocks w, w + 4, .. of A (RW = M / 64 of them) and all 64 columns; its A fragments stay in
+// registers for the whole launch, so per tile only X comes from LDS (one fragment per column
+// block and k step).
 template <int K>
 __global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
                                                     uint64_t cmask, TMask<K> tm, const double* __restrict__ at,
-                                                    int member_fast) {
-    constexpr int D = 1 << K, M = 2 * D, P = MFMA_PITCH, RT = M / 16;
-    extern __shared__ double xs[];  // M x P
-    const uint64_t g0 = (uint64_t)blockIdx.x * MFMA_NG;
-    const int tid = threadIdx.x;
-    // element e of the tile: member m, column c; member-fast order keeps consecutive lanes on
+                                                    int member_fast, uint64_t tiles) {
+    constexpr int D = 1 << K, M = 2 * D, P = MFMA_PITCH, KS = M / 4, RW = M / 64;
+    extern __shared__ double xs[];  // M x P; the pitch padding of rows 0..3 holds the 64 group bases
+    uint64_t* const xw = reinterpret_cast<uint64_t*>(xs);
+    auto gbase = [&](int c) -> uint64_t& { return xw[(c >> 4) * P + MFMA_NG + (c & 15)]; };
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, l16 = lane & 15;
+    double a[RW][KS];  // A[(w + 4 j) 16 + l16][4 ks + kq]
+#pragma unroll
+    for (int j = 0; j < RW; j++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) a[j][ks] = at[(size_t)(4 * ks + kq) * M + (wave + 4 * j) * 16 + l16];
+    // element e of a tile: member m, column c; member-fast order keeps consecutive lanes on
     // consecutive addresses when a target is among the lowest qubits, column-fast otherwise
     auto where = [&](int e, int& m, int& c) {
         if (member_fast) {
@@ -130,40 +140,54 @@ __global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, u
             m = e / MFMA_NG;
         }
     };
-    for (int e = tid; e < D * MFMA_NG; e += 256) {
-        int m, c;
-        where(e, m, c);
-        const uint64_t g = g0 + (uint64_t)c;
-        double2 v = make_double2(0.0, 0.0);
-        if (g < ngroups) v = psi[insert_zeros(g, sp) | cmask | tm.m[m]];
-        xs[m * P + c] = v.x;
-        xs[(m + D) * P + c] = v.y;
-    }
-    __syncthreads();
-    const int wave = tid >> 6, lane = tid & 63;
-    const int col = wave * 16 + (lane & 15), kq = lane >> 4;
-    dx4 acc[RT];
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t g0 = t * MFMA_NG;
+        if (tid < MFMA_NG) {
+            const uint64_t g = g0 + (uint64_t)tid;
+            gbase(tid) = g < ngroups ? (insert_zeros(g, sp) | cmask) : ~0ull;
+        }
+        __syncthreads();
+        for (int e = tid; e < D * MFMA_NG; e += 256) {
+            int m, c;
+            where(e, m, c);
+            const uint64_t gb = gbase(c);
+            double2 v = make_double2(0.0, 0.0);
+            if (gb != ~0ull) v = psi[gb | tm.m[m]];
+            xs[m * P + c] = v.x;
+            xs[(m + D) * P + c] = v.y;
+        }
+        __syncthreads();
+        dx4 acc[RW][4];
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++) acc[rt] = dx4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-    for (int k0 = 0; k0 < M; k0 += 4) {
-        const double b = xs[(k0 + kq) * P + col];
-        const double* arow = at + (size_t)(k0 + kq) * M + (lane & 15);  // at[k][r] = A[r][k]
+        for (int j = 0; j < RW; j++)
 #pragma unroll
-        for (int rt = 0; rt < RT; rt++)
-            acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[rt * 16], b, acc[rt], 0, 0, 0);
-    }
-    __syncthreads();  // every wavefront has read its columns of X
+            for (int ct = 0; ct < 4; ct++) acc[j][ct] = dx4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++)
+        for (int ks = 0; ks < KS; ks++) {
+            double b[4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) xs[(rt * 16 + 4 * v + kq) * P + col] = acc[rt][v];
-    __syncthreads();
-    for (int e = tid; e < D * MFMA_NG; e += 256) {
-        int m, c;
-        where(e, m, c);
-        const uint64_t g = g0 + (uint64_t)c;
-        if (g < ngroups) psi[insert_zeros(g, sp) | cmask | tm.m[m]] = make_double2(xs[m * P + c], xs[(m + D) * P + c]);
+            for (int ct = 0; ct < 4; ct++) b[ct] = xs[(4 * ks + kq) * P + ct * 16 + l16];
+#pragma unroll
+            for (int j = 0; j < RW; j++)
+#pragma unroll
+                for (int ct = 0; ct < 4; ct++)
+                    acc[j][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j][ks], b[ct], acc[j][ct], 0, 0, 0);
+        }
+        __syncthreads();  // every wavefront has read X
+#pragma unroll
+        for (int j = 0; j < RW; j++)
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) xs[((wave + 4 * j) * 16 + 4 * v + kq) * P + ct * 16 + l16] = acc[j][ct][v];
+        __syncthreads();
+        for (int e = tid; e < D * MFMA_NG; e += 256) {
+            int m, c;
+            where(e, m, c);
+            const uint64_t gb = gbase(c);
+            if (gb != ~0ull) psi[gb | tm.m[m]] = make_double2(xs[m * P + c], xs[(m + D) * P + c]);
+        }
+        __syncthreads();  // the next tile rewrites xs and gbase
     }
 }
 
@@ -347,11 +371,14 @@ static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uin
     const int member_fast = lowest < lowest_free ? 1 : 0;
     const size_t lds = (size_t)M * MFMA_PITCH * sizeof(double);
     if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_multi_mfma<K>), s.device);
-    const uint64_t blocks = (ngroups + MFMA_NG - 1) / MFMA_NG;
-    if (blocks > 0xffffffffull) throw Error(QFVM_ERR_VALUE, "dense gate grid too large");
+    const uint64_t tiles = (ngroups + MFMA_NG - 1) / MFMA_NG;
+    int cus = 0;
+    QFVM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device));
+    const uint64_t per_cu = (160 * 1024) / lds;  // workgroups that fit the LDS of one CU
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 1) * std::max<uint64_t>(per_cu, 1));
     KernelScope ks(s, "multi_mfma", (double)ngroups * D * 2.0 * sizeof(double2));
-    hipLaunchKernelGGL((k_multi_mfma<K>), dim3((unsigned)blocks), dim3(256), lds, s.stream,
-                       reinterpret_cast<double2*>(s.psi), ngroups, sp, g.cmask, tm, dat, member_fast);
+    hipLaunchKernelGGL((k_multi_mfma<K>), dim3(grid), dim3(256), lds, s.stream,
+                       reinterpret_cast<double2*>(s.psi), ngroups, sp, g.cmask, tm, dat, member_fast, tiles);
     QFVM_LAUNCHED();
     QFVM_HIP(hipStreamSynchronize(s.stream));  // the host matrix copy must outlive the async copy
 }
