@@ -311,6 +311,46 @@ def batch_leg(count: int, workers: int, graphs: bool = False):
     }
 
 
+def dense_leg(n: int = 28, reps: int = 3):
+    """Dense complex128 gates on k = 5 and 6 targets (seeded Haar-random unitaries, PCG64 seed
+    20250006; targets the k lowest, the k highest, or qubits 8 .. 8+k-1) on an n-qubit state, on
+    the FP64 matrix cores (option mfma = 1) and on the vector-ALU kernels (mfma = 0). Per gate:
+    mean HIP-event duration, algorithmic bytes 2 * 16 * 2^n and flops 8 * 2^k * 2^n."""
+    from pyquafu_amd import _native
+    from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
+
+    rng = np.random.Generator(np.random.PCG64(20250006))
+    dev = _native.DeviceState(n, "c128")
+    dev.set_profiling(True)
+    out = {"workload": f"{n}q c128, one dense k-target gate", "rows": []}
+    for k in (5, 6):
+        d = 1 << k
+        a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        q, r = np.linalg.qr(a)
+        u = q * (np.diag(r) / np.abs(np.diag(r)))
+        for where, tg in (("low", list(range(k))), ("mid", list(range(8, 8 + k))), ("high", list(range(n - k, n)))):
+            b = ProgramBuilder()
+            b.add(MULTI, tg, matrix=u)
+            prog = b.program(n_qubits=n)
+            for mfma in (1, 0):
+                dev.set_option("mfma", mfma)
+                dev.init_zero()
+                dev.apply(prog)
+                dev.synchronize()
+                dev.profile(reset=True)
+                for _ in range(reps):
+                    dev.apply(prog)
+                dev.synchronize()
+                prof = dev.profile(reset=True)
+                name = "multi_mfma" if mfma else ("multi_reg" if k <= 5 else "multi_lds")
+                ms = prof[name]["ms"] / prof[name]["launches"]
+                out["rows"].append({"k": k, "targets": where, "kernel": name, "ms": round(ms, 3),
+                                    "GBps": round(2 * 16 * 2 ** n / ms / 1e6, 1),
+                                    "TFLOPs": round(8 * d * 2 ** n / ms / 1e9, 2)})
+    dev.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +371,7 @@ def main(argv=None) -> int:
     ap.add_argument("--adjoint-n", type=int, default=24)
     ap.add_argument("--adjoint-layers", type=int, default=8)
     ap.add_argument("--no-qft", action="store_true", help="skip the 33-qubit QFT precision leg")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense k-target gate leg (matrix cores)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-circuit leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=4)
@@ -443,6 +484,11 @@ def main(argv=None) -> int:
         out["precision"] = qft_leg(args.qft_n, args.qft_shots)
     else:
         out["precision"] = None
+    if not args.no_dense:
+        print("[bench] dense gate leg: k = 5, 6 on 28 qubits", file=sys.stderr, flush=True)
+        out["dense"] = dense_leg()
+    else:
+        out["dense"] = None
     if not args.no_cpu:
         per_layer = n + n // 2 + (n - 1) // 2 + args.crz
         print(f"[bench] CPU baseline: {per_layer} gates on the oracle", file=sys.stderr, flush=True)
