@@ -218,3 +218,26 @@ def test_hamiltonian_matrices_against_reference_fixtures():
 def test_compile_ops_for_applyop():
     prog = compile_ops([g.RXGate(1, 0.5), g.CZGate(0, 1)], 2)
     assert prog.ops["code"].tolist() == [P.RX, P.CZ]
+
+
+def test_batch_partition_is_deterministic_and_balanced():
+    """Graph-mode batches give each worker slot a fixed share (longest first onto the least-loaded
+    slot), so a circuit evaluated again lands on the handle that holds its graph."""
+    from types import SimpleNamespace
+
+    from pyquafu_amd.batch import BatchRunner
+
+    rng = np.random.default_rng(5)
+    items = [SimpleNamespace(n=int(rng.integers(10, 23)), call=SimpleNamespace(cp=SimpleNamespace(n_ops=int(rng.integers(20, 400)))))
+             for _ in range(97)]
+    runner = BatchRunner(dtype="c64", workers=4, graphs=True)
+    try:
+        parts = runner._partition(items)  # pylint: disable=protected-access
+        assert parts == runner._partition(items)  # pylint: disable=protected-access
+        assert sorted(k for p in parts for k in p) == list(range(len(items)))
+        cost = [(1 << it.n) * (it.call.cp.n_ops + 8) for it in items]
+        loads = [sum(cost[k] for k in p) for p in parts]
+        # LPT: no slot exceeds the mean load by more than the largest single item
+        assert max(loads) <= sum(loads) / len(loads) + max(cost)
+    finally:
+        runner.close()
