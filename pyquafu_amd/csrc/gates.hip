@@ -108,21 +108,29 @@ __global__ void __launch_bounds__(256) k_multi_reg(V2<R>* __restrict__ psi, uint
 //   D (16 x 16): lane l holds D[4 v + l / 16][l % 16] in accumulator element v = 0..3
 //   (measured on gfx950 with one-hot operands).
 typedef double dx4 __attribute__((ext_vector_type(4)));
-constexpr int MFMA_NG = 64;               // groups per workgroup tile
-constexpr int MFMA_PITCH = MFMA_NG + 16;  // LDS row pitch in doubles (rows k and k + 1 in other banks)
+constexpr int MFMA_NG = 64;  // groups per workgroup tile
+// LDS row pitch in doubles: odd, so that the member-fast gather (consecutive lanes on consecutive
+// rows) hits 32 distinct bank pairs; 79 keeps two K = 6 workgroups within the 160 KiB of a CU
+constexpr int mfma_pitch(int K) { return K == 6 ? 79 : 81; }
 
 // Persistent: each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ... Wavefront w owns the
 // row blocks w, w + 4, .. of A (RW = M / 64 of them) and all 64 columns; its A fragments stay in
 // registers for the whole launch, so per tile only X comes from LDS (one fragment per column
 // block and k step).
 template <int K>
-__global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
+__global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
                                                     uint64_t cmask, TMask<K> tm, const double* __restrict__ at,
                                                     int member_fast, uint64_t tiles) {
-    constexpr int D = 1 << K, M = 2 * D, P = MFMA_PITCH, KS = M / 4, RW = M / 64;
-    extern __shared__ double xs[];  // M x P; the pitch padding of rows 0..3 holds the 64 group bases
+    constexpr int D = 1 << K, M = 2 * D, P = mfma_pitch(K), KS = M / 4, RW = M / 64;
+    // M x P doubles; the pitch padding (PAD slots per row) of the first rows holds the tile's 64
+    // group bases, that of the next rows the D member offsets, so gather and scatter look both up
+    constexpr int PAD = P - MFMA_NG, GROWS = (MFMA_NG + PAD - 1) / PAD;
+    static_assert(GROWS + (D + PAD - 1) / PAD <= M, "lookup tables exceed the padding");
+    extern __shared__ double xs[];
     uint64_t* const xw = reinterpret_cast<uint64_t*>(xs);
-    auto gbase = [&](int c) -> uint64_t& { return xw[(c >> 4) * P + MFMA_NG + (c & 15)]; };
+    auto gbase = [&](int c) -> uint64_t& { return xw[(c / PAD) * P + MFMA_NG + c % PAD]; };
+    auto moff = [&](int m) -> uint64_t& { return xw[(GROWS + m / PAD) * P + MFMA_NG + m % PAD]; };
+    constexpr int ITER = D * MFMA_NG / 256;  // elements per thread and tile (8 or 16)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, l16 = lane & 15;
     double a[RW][KS];  // A[(w + 4 j) 16 + l16][4 ks + kq]
 #pragma unroll
@@ -140,6 +148,7 @@ __global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, u
             m = e / MFMA_NG;
         }
     };
+    if (tid < D) moff(tid) = tm.m[tid];  // read before the first tile's barrier
     for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint64_t g0 = t * MFMA_NG;
         if (tid < MFMA_NG) {
@@ -147,14 +156,25 @@ __global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, u
             gbase(tid) = g < ngroups ? (insert_zeros(g, sp) | cmask) : ~0ull;
         }
         __syncthreads();
-        for (int e = tid; e < D * MFMA_NG; e += 256) {
-            int m, c;
-            where(e, m, c);
-            const uint64_t gb = gbase(c);
-            double2 v = make_double2(0.0, 0.0);
-            if (gb != ~0ull) v = psi[gb | tm.m[m]];
-            xs[m * P + c] = v.x;
-            xs[(m + D) * P + c] = v.y;
+        // loads issued CH at a time, then written to LDS (K = 6 holds 128 registers of A, so fewer)
+        constexpr int CH = K == 6 ? 4 : 8;
+#pragma unroll
+        for (int i0 = 0; i0 < ITER; i0 += CH) {
+            double2 v[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int m, c;
+                where(tid + 256 * (i0 + i), m, c);
+                const uint64_t gb = gbase(c);
+                v[i] = gb != ~0ull ? psi[gb | moff(m)] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int m, c;
+                where(tid + 256 * (i0 + i), m, c);
+                xs[m * P + c] = v[i].x;
+                xs[(m + D) * P + c] = v[i].y;
+            }
         }
         __syncthreads();
         dx4 acc[RW][4];
@@ -181,11 +201,12 @@ __global__ void __launch_bounds__(256) k_multi_mfma(double2* __restrict__ psi, u
 #pragma unroll
                 for (int v = 0; v < 4; v++) xs[((wave + 4 * j) * 16 + 4 * v + kq) * P + ct * 16 + l16] = acc[j][ct][v];
         __syncthreads();
-        for (int e = tid; e < D * MFMA_NG; e += 256) {
+#pragma unroll
+        for (int i = 0; i < ITER; i++) {
             int m, c;
-            where(e, m, c);
+            where(tid + 256 * i, m, c);
             const uint64_t gb = gbase(c);
-            if (gb != ~0ull) psi[gb | tm.m[m]] = make_double2(xs[m * P + c], xs[(m + D) * P + c]);
+            if (gb != ~0ull) psi[gb | moff(m)] = make_double2(xs[m * P + c], xs[(m + D) * P + c]);
         }
         __syncthreads();  // the next tile rewrites xs and gbase
     }
@@ -369,7 +390,7 @@ static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uin
     while (lowest_free < s.n && (((g.cmask >> lowest_free) & 1) || std::find(g.targets.begin(), g.targets.end(), lowest_free) != g.targets.end()))
         lowest_free++;
     const int member_fast = lowest < lowest_free ? 1 : 0;
-    const size_t lds = (size_t)M * MFMA_PITCH * sizeof(double);
+    const size_t lds = (size_t)M * mfma_pitch(K) * sizeof(double);
     if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_multi_mfma<K>), s.device);
     const uint64_t tiles = (ngroups + MFMA_NG - 1) / MFMA_NG;
     int cus = 0;
