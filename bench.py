@@ -312,7 +312,7 @@ def batch_leg(count: int, workers: int, graphs: bool = False):
 
 
 def dense_leg(n: int = 28, reps: int = 3):
-    """Dense complex128 gates on k = 5 and 6 targets (seeded Haar-random unitaries, PCG64 seed
+    """Dense complex128 gates on k = 4, 5 and 6 targets (seeded Haar-random unitaries, PCG64 seed
     20250006; targets the k lowest, the k highest, or qubits 8 .. 8+k-1) on an n-qubit state, on
     the FP64 matrix cores (option mfma = 1) and on the vector-ALU kernels (mfma = 0). Per gate:
     mean HIP-event duration, algorithmic bytes 2 * 16 * 2^n and flops 8 * 2^k * 2^n."""
@@ -323,7 +323,7 @@ def dense_leg(n: int = 28, reps: int = 3):
     dev = _native.DeviceState(n, "c128")
     dev.set_profiling(True)
     out = {"workload": f"{n}q c128, one dense k-target gate", "rows": []}
-    for k in (5, 6):
+    for k in (4, 5, 6):
         d = 1 << k
         a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
         q, r = np.linalg.qr(a)
@@ -485,7 +485,7 @@ def main(argv=None) -> int:
     else:
         out["precision"] = None
     if not args.no_dense:
-        print("[bench] dense gate leg: k = 5, 6 on 28 qubits", file=sys.stderr, flush=True)
+        print("[bench] dense gate leg: k = 4, 5, 6 on 28 qubits", file=sys.stderr, flush=True)
         out["dense"] = dense_leg()
     else:
         out["dense"] = None

@@ -96,13 +96,13 @@ __global__ void __launch_bounds__(256) k_multi_reg(V2<R>* __restrict__ psi, uint
     }
 }
 
-// Dense 2^K x 2^K complex128 matrix on the FP64 matrix cores (K = 5, 6). The gate is a real GEMM:
+// Dense 2^K x 2^K complex128 matrix on the FP64 matrix cores (K = 4, 5, 6). The gate is a real GEMM:
 // with X the tile's amplitudes as columns (rows 0..D-1 real parts, D..2D-1 imaginary parts) and
 // A = [[Ur, -Ui], [Ui, Ur]], Y = A X. One workgroup (4 wavefronts) takes NG = 64 groups: gather
 // into LDS, the wavefronts multiply with v_mfma_f64_16x16x4f64 (A held in registers, X from LDS),
-// results back through LDS, scatter. At K >= 5 a dense gate moves 2 x 16 B per amplitude
-// but costs 16 x 2^K flops per amplitude (>= 16 flop/B, past the FP64 ridge of ~9.8 flop/B), so it
-// is compute-bound and the matrix cores carry the arithmetic.
+// results back through LDS, scatter. A dense gate moves 2 x 16 B per amplitude and costs 8 x 2^K
+// flops per amplitude: 4, 8 and 16 flop/B for K = 4, 5, 6 against the FP64 ridge of ~9.8 flop/B,
+// so from K = 5 on the arithmetic decides and the matrix cores carry it.
 // Operand layouts of v_mfma_f64_16x16x4f64 (one 16x16 output block per wavefront):
 //   A (16 x 4): lane l supplies A[l % 16][l / 16];  B (4 x 16): lane l supplies B[l / 16][l % 16];
 //   D (16 x 16): lane l holds D[4 v + l / 16][l % 16] in accumulator element v = 0..3
@@ -113,15 +113,17 @@ constexpr int MFMA_NG = 64;  // groups per workgroup tile
 // rows) hits 32 distinct bank pairs; 79 keeps two K = 6 workgroups within the 160 KiB of a CU
 constexpr int mfma_pitch(int K) { return K == 6 ? 79 : 81; }
 
-// Persistent: each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ... Wavefront w owns the
-// row blocks w, w + 4, .. of A (RW = M / 64 of them) and all 64 columns; its A fragments stay in
-// registers for the whole launch, so per tile only X comes from LDS (one fragment per column
-// block and k step).
+// Persistent: each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ... For K >= 5 wavefront
+// w owns the row blocks w, w + 4, .. of A (M / 64 of them) and all four 16-column blocks; for K = 4
+// (two row blocks) it owns both row blocks and column block w. Its A fragments stay in registers for
+// the whole launch, so per tile only X comes from LDS (one fragment per column block and k step).
 template <int K>
 __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
                                                     uint64_t cmask, TMask<K> tm, const double* __restrict__ at,
                                                     int member_fast, uint64_t tiles) {
-    constexpr int D = 1 << K, M = 2 * D, P = mfma_pitch(K), KS = M / 4, RW = M / 64;
+    constexpr int D = 1 << K, M = 2 * D, P = mfma_pitch(K), KS = M / 4;
+    constexpr bool ROWSPLIT = M >= 64;
+    constexpr int NR = ROWSPLIT ? M / 64 : M / 16, NC = ROWSPLIT ? 4 : 1;  // blocks per wavefront
     // M x P doubles; the pitch padding (PAD slots per row) of the first rows holds the tile's 64
     // group bases, that of the next rows the D member offsets, so gather and scatter look both up
     constexpr int PAD = P - MFMA_NG, GROWS = (MFMA_NG + PAD - 1) / PAD;
@@ -130,13 +132,15 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __r
     uint64_t* const xw = reinterpret_cast<uint64_t*>(xs);
     auto gbase = [&](int c) -> uint64_t& { return xw[(c / PAD) * P + MFMA_NG + c % PAD]; };
     auto moff = [&](int m) -> uint64_t& { return xw[(GROWS + m / PAD) * P + MFMA_NG + m % PAD]; };
-    constexpr int ITER = D * MFMA_NG / 256;  // elements per thread and tile (8 or 16)
+    constexpr int ITER = D * MFMA_NG / 256;  // elements per thread and tile (4, 8 or 16)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, l16 = lane & 15;
-    double a[RW][KS];  // A[(w + 4 j) 16 + l16][4 ks + kq]
+    auto rowb = [&](int j) { return ROWSPLIT ? wave + 4 * j : j; };
+    auto colb = [&](int c) { return ROWSPLIT ? c : wave; };
+    double a[NR][KS];  // A[rowb(j) 16 + l16][4 ks + kq]
 #pragma unroll
-    for (int j = 0; j < RW; j++)
+    for (int j = 0; j < NR; j++)
 #pragma unroll
-        for (int ks = 0; ks < KS; ks++) a[j][ks] = at[(size_t)(4 * ks + kq) * M + (wave + 4 * j) * 16 + l16];
+        for (int ks = 0; ks < KS; ks++) a[j][ks] = at[(size_t)(4 * ks + kq) * M + rowb(j) * 16 + l16];
     // element e of a tile: member m, column c; member-fast order keeps consecutive lanes on
     // consecutive addresses when a target is among the lowest qubits, column-fast otherwise
     auto where = [&](int e, int& m, int& c) {
@@ -157,7 +161,7 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __r
         }
         __syncthreads();
         // loads issued CH at a time, then written to LDS (K = 6 holds 128 registers of A, so fewer)
-        constexpr int CH = K == 6 ? 4 : 8;
+        constexpr int CH = K == 6 ? 4 : (ITER < 8 ? ITER : 8);
 #pragma unroll
         for (int i0 = 0; i0 < ITER; i0 += CH) {
             double2 v[CH];
@@ -177,29 +181,29 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __r
             }
         }
         __syncthreads();
-        dx4 acc[RW][4];
+        dx4 acc[NR][NC];
 #pragma unroll
-        for (int j = 0; j < RW; j++)
+        for (int j = 0; j < NR; j++)
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) acc[j][ct] = dx4{0.0, 0.0, 0.0, 0.0};
+            for (int ct = 0; ct < NC; ct++) acc[j][ct] = dx4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            double b[4];
+            double b[NC];
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) b[ct] = xs[(4 * ks + kq) * P + ct * 16 + l16];
+            for (int ct = 0; ct < NC; ct++) b[ct] = xs[(4 * ks + kq) * P + colb(ct) * 16 + l16];
 #pragma unroll
-            for (int j = 0; j < RW; j++)
+            for (int j = 0; j < NR; j++)
 #pragma unroll
-                for (int ct = 0; ct < 4; ct++)
+                for (int ct = 0; ct < NC; ct++)
                     acc[j][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j][ks], b[ct], acc[j][ct], 0, 0, 0);
         }
         __syncthreads();  // every wavefront has read X
 #pragma unroll
-        for (int j = 0; j < RW; j++)
+        for (int j = 0; j < NR; j++)
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++)
+            for (int ct = 0; ct < NC; ct++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) xs[((wave + 4 * j) * 16 + 4 * v + kq) * P + ct * 16 + l16] = acc[j][ct][v];
+                for (int v = 0; v < 4; v++) xs[(rowb(j) * 16 + 4 * v + kq) * P + colb(ct) * 16 + l16] = acc[j][ct][v];
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < ITER; i++) {
@@ -393,10 +397,10 @@ static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uin
     const size_t lds = (size_t)M * mfma_pitch(K) * sizeof(double);
     if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_multi_mfma<K>), s.device);
     const uint64_t tiles = (ngroups + MFMA_NG - 1) / MFMA_NG;
-    int cus = 0;
+    int cus = 0, per_cu = 0;  // resident workgroups: registers and LDS both count
     QFVM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device));
-    const uint64_t per_cu = (160 * 1024) / lds;  // workgroups that fit the LDS of one CU
-    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 1) * std::max<uint64_t>(per_cu, 1));
+    QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_multi_mfma<K>), 256, lds));
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1));
     KernelScope ks(s, "multi_mfma", (double)ngroups * D * 2.0 * sizeof(double2));
     hipLaunchKernelGGL((k_multi_mfma<K>), dim3(grid), dim3(256), lds, s.stream,
                        reinterpret_cast<double2*>(s.psi), ngroups, sp, g.cmask, tm, dat, member_fast, tiles);
@@ -415,8 +419,10 @@ static void multi_typed(State& s, const Gate& g) {
     const InsertSpec sp = make_spec(pos);
     const uint64_t ngroups = s.size >> pos.size();
     if constexpr (sizeof(R) == 8) {
-        if (s.mfma && (k == 5 || k == 6)) {
-            if (k == 5)
+        if (s.mfma && k >= 4 && k <= 6) {
+            if (k == 4)
+                multi_mfma_launch<4>(s, g, sp, ngroups);
+            else if (k == 5)
                 multi_mfma_launch<5>(s, g, sp, ngroups);
             else
                 multi_mfma_launch<6>(s, g, sp, ngroups);

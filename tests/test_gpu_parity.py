@@ -502,10 +502,10 @@ def test_full_size_round_trip_30_qubits():
 
 
 # ------------------------------------------------------------------ dense gates on the matrix cores
-@pytest.mark.parametrize("k,where,nc", [(5, "low", 0), (5, "high", 1), (5, "mixed", 2), (6, "low", 1),
-                                        (6, "high", 0), (6, "mixed", 2)])
+@pytest.mark.parametrize("k,where,nc", [(4, "low", 0), (4, "mixed", 1), (5, "low", 0), (5, "high", 1),
+                                        (5, "mixed", 2), (6, "low", 1), (6, "high", 0), (6, "mixed", 2)])
 def test_dense_mfma_matches_oracle(k, where, nc):
-    """complex128 dense gates on 5 and 6 targets run on v_mfma_f64_16x16x4f64 (option mfma, the
+    """complex128 dense gates on 4 to 6 targets run on v_mfma_f64_16x16x4f64 (option mfma, the
     default); against the oracle and against the VALU kernels (mfma = 0), with the targets among
     the lowest qubits (member-fast tile order), the highest, or spread, and with controls."""
     from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
