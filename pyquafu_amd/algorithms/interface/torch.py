@@ -60,18 +60,20 @@ class ExpectationFunction(torch.autograd.Function):
     """<H> of a parameterised circuit; gradients by the engine's adjoint sweep."""
 
     @staticmethod
-    def forward(ctx, params, circ, hamiltonian, precision):  # pylint: disable=arguments-differ
+    def forward(ctx, params, circ, hamiltonian, precision, checkpoints=0):  # pylint: disable=arguments-differ
         circ._update_params(params.detach().cpu().numpy().tolist())  # pylint: disable=protected-access
-        grads, e = grad_adjoint(circ, hamiltonian, precision=precision, return_expectation=True)
+        grads, e = grad_adjoint(circ, hamiltonian, precision=precision, return_expectation=True,
+                                checkpoints=checkpoints)
         ctx.save_for_backward(torch.tensor(grads, dtype=torch.float64))
         return torch.tensor(e, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, grad_out):  # pylint: disable=arguments-differ
         (grads,) = ctx.saved_tensors
-        return grad_out * grads, None, None, None
+        return grad_out * grads, None, None, None, None
 
 
-def expectation(circ, hamiltonian, params: torch.Tensor, precision: str = "c128") -> torch.Tensor:
-    """Differentiable <H>(params); params follow circ.variables' order."""
-    return ExpectationFunction.apply(params, circ, hamiltonian, precision)
+def expectation(circ, hamiltonian, params: torch.Tensor, precision: str = "c128", checkpoints: int = 0) -> torch.Tensor:
+    """Differentiable <H>(params); params follow circ.variables' order. `checkpoints` > 0 keeps that
+    many forward-state copies for the reverse sweep (see grad_adjoint)."""
+    return ExpectationFunction.apply(params, circ, hamiltonian, precision, checkpoints)

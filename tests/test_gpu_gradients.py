@@ -240,6 +240,12 @@ def test_torch_expectation_function_backward():
     psi, _ = dense.run_dense(compile_circuit(qc), n)
     terms, coeffs = hamiltonian_terms(ham)
     assert abs(loss.item() / 3.0 - og.expectation(psi, n, terms, coeffs)) < 1e-12
+    # the same through forward-state checkpoints
+    params2 = torch.tensor(values, dtype=torch.float64, requires_grad=True)
+    loss2 = 3.0 * expectation(qc, ham, params2, checkpoints=3)
+    loss2.backward()
+    assert abs(loss2.item() - loss.item()) < 1e-12
+    assert np.abs(params2.grad.numpy() - params.grad.numpy()).max() < 1e-12
 
 
 def test_execute_circuits_jacobians_agree():
