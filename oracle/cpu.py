@@ -75,6 +75,10 @@ def lib():
         L.qfvm_sample.argtypes = [vp, vp, i64, vp]
         L.qfvm_sample_sequential.argtypes = [vp, vp, i64, vp]
         L.qfvm_norm2.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.qfvm_oracle_data.restype = ctypes.c_void_p
+        L.qfvm_oracle_data.argtypes = [vp]
+        L.qfvm_adjoint.argtypes = [vp, ctypes.POINTER(_Program), vp, vp, i64, vp, vp, vp, vp, vp, i64, vp,
+                                   ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -132,6 +136,13 @@ class CpuState:
         _check(lib().qfvm_store(self.h, out.ctypes.data, out.size))
         return out
 
+    def view(self) -> np.ndarray:
+        """The host state itself (complex128, no copy); valid while this object lives and until the
+        next call that reallocates it."""
+        ptr = lib().qfvm_oracle_data(self.h)
+        buf = (ctypes.c_double * (2 << self.n)).from_address(ptr)
+        return np.frombuffer(buf, dtype=np.complex128)
+
     def apply(self, prog) -> None:
         p, keep = _as_program(prog)
         _check(lib().qfvm_apply(self.h, ctypes.byref(p)))
@@ -183,6 +194,34 @@ class CpuState:
         out = np.empty(u.size, dtype=np.uint64)
         _check(lib().qfvm_sample_sequential(self.h, u.ctypes.data, u.size, out.ctypes.data))
         return out
+
+    def adjoint(self, prog, grad_ops, grad_gens: str, terms, coeffs):
+        """grad_adjoint restated on the oracle's kernels (qfvm_adjoint in qfvm_cpu.cpp); terms as
+        for `expect`, one complex coefficient per term. Returns (grads, <H>)."""
+        p, keep = _as_program(prog)
+        ops = np.ascontiguousarray(grad_ops, dtype=np.int64)
+        gens = np.frombuffer(grad_gens.encode() + b"\0", dtype=np.uint8)
+        chars, pos, off, ln = [], [], [], []
+        for s, q in terms:
+            off.append(len(pos))
+            ln.append(len(q))
+            chars.extend(s.encode())
+            pos.extend(int(x) for x in q)
+        c = np.array(chars + [0], dtype=np.uint8)
+        ps = np.array(pos + [0], dtype=np.int32)
+        o = np.array(off + [0], dtype=np.int64)
+        l_ = np.array(ln + [0], dtype=np.int32)
+        cf = np.asarray(coeffs, dtype=np.complex128).reshape(-1)
+        cfa = np.zeros(2 * max(cf.size, 1))
+        cfa[0: 2 * cf.size: 2] = cf.real
+        cfa[1: 2 * cf.size: 2] = cf.imag
+        grads = np.zeros(max(ops.size, 1))
+        e = ctypes.c_double(0.0)
+        _check(lib().qfvm_adjoint(self.h, ctypes.byref(p), ops.ctypes.data, gens.ctypes.data, ops.size,
+                                  c.ctypes.data, ps.ctypes.data, o.ctypes.data, l_.ctypes.data, cfa.ctypes.data,
+                                  len(off), grads.ctypes.data, ctypes.byref(e)))
+        del keep
+        return grads[: ops.size], e.value
 
     def norm2(self) -> float:
         v = ctypes.c_double()

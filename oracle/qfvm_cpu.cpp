@@ -765,6 +765,172 @@ int qfvm_synchronize(qfvm_handle h) {
 }
 
 /*
+ * grad_adjoint (quafu/algorithms/gradients/gradiant.py:144-173) on the oracle's kernels, step for
+ * step, with the HIP library's argument layout (include/qfvm.h qfvm_adjoint):
+ *   forward: every op of prog on the state (SVSimulator.run);
+ *   phi = psi; lam = H psi (_apply_hamil, simulator.py:78-88: per term, a copy with the term's
+ *   X / Y / Z gates applied one by one, times the coefficient, summed);
+ *   for op i from the last down to the first differentiated one: phi <- U_i^dagger phi; for a
+ *   differentiated op, mu = projCtrl(dU phi) with dU = -1/2 i P U on the target alone (grad_gate,
+ *   gradiant.py:96-141) and grad = Re(2 <lam, mu>); then lam <- U_i^dagger lam.
+ * U^dagger is the op's own inverse as op.dagger() builds it: the conjugate transpose of a matrix,
+ * the negated angle of a rotation / phase, S <-> SDG, T <-> TDG, the Paulis, H, CX, CZ and CCX
+ * themselves. On return the state holds phi just before the first differentiated op.
+ */
+int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_ops, const char* grad_gen,
+                 int64_t n_grads, const char* chars, const int32_t* pos, const int64_t* off,
+                 const int32_t* len, const double* coef, int64_t n_terms, double* grads, double* expectation) {
+    return guarded([&] {
+        need_handle(h);
+        if (!prog || prog->n_ops < 0) throw QfvmError(QFVM_ERR_VALUE, "null program");
+        std::vector<int64_t> slot_of((size_t)prog->n_ops, -1);
+        int64_t begin = prog->n_ops;
+        for (int64_t g = 0; g < n_grads; g++) {
+            const int64_t k = grad_ops[g];
+            if (k < 0 || k >= prog->n_ops) throw QfvmError(QFVM_ERR_INDEX, "gradient op out of range");
+            slot_of[(size_t)k] = g;
+            begin = std::min(begin, k);
+        }
+        for (int64_t k = 0; k < prog->n_ops; k++) apply_gate(h, prog, prog->ops[k]);
+        const std::vector<cd> psi = h->data;
+        // lam = H psi
+        std::vector<cd> lam(h->size, cd(0.0, 0.0));
+        for (int64_t t = 0; t < n_terms; t++) {
+            h->data = psi;
+            for (int32_t k = 0; k < len[t]; k++) {
+                const int q = pos[off[t] + k];
+                if (q < 0 || q >= h->n) throw QfvmError(QFVM_ERR_INDEX, "pauli qubit out of range");
+                switch (chars[off[t] + k]) {
+                    case 'X': h->apply_x(q); break;
+                    case 'Y': h->apply_y(q); break;
+                    case 'Z': h->apply_z(q); break;
+                    case 'I': break;
+                    default: throw QfvmError(QFVM_ERR_VALUE, "Pauli letters are I, X, Y, Z");
+                }
+            }
+            const cd c(coef[2 * t], coef[2 * t + 1]);
+            const cd* v = h->data.data();
+            cd* l = lam.data();
+#pragma omp parallel for
+            for (int64_t j = 0; j < (int64_t)h->size; j++) l[j] += v[j] * c;
+        }
+        if (expectation) {
+            double e = 0.0;
+            const cd* a = psi.data();
+            const cd* l = lam.data();
+#pragma omp parallel for reduction(+ : e)
+            for (int64_t j = 0; j < (int64_t)h->size; j++) e += std::real(std::conj(a[j]) * l[j]);
+            *expectation = e;
+        }
+        // one-op programs for U^dagger and dU
+        std::vector<int32_t> idx;
+        std::vector<double> mat;
+        auto dagger_of = [&](const qfvm_op& op, qfvm_op& d) {
+            d = op;
+            idx.assign(prog->idx + op.q_off, prog->idx + op.q_off + op.n_qubits);
+            d.q_off = 0;
+            switch (op.code) {
+                case QFVM_OP_GENERAL:
+                case QFVM_OP_MULTI: {
+                    const int nt = op.n_qubits - op.n_ctrl;
+                    const size_t dim = 1ull << nt;
+                    std::vector<cd> m = op_matrix(prog, op, dim);
+                    mat.assign(2 * dim * dim, 0.0);
+                    for (size_t r = 0; r < dim; r++)
+                        for (size_t c = 0; c < dim; c++) {
+                            const cd z = std::conj(m[c * dim + r]);
+                            mat[2 * (r * dim + c)] = z.real();
+                            mat[2 * (r * dim + c) + 1] = z.imag();
+                        }
+                    d.m_off = 0;
+                    break;
+                }
+                case QFVM_OP_S: d.code = QFVM_OP_SDG; break;
+                case QFVM_OP_SDG: d.code = QFVM_OP_S; break;
+                case QFVM_OP_T: d.code = QFVM_OP_TDG; break;
+                case QFVM_OP_TDG: d.code = QFVM_OP_T; break;
+                case QFVM_OP_P:
+                case QFVM_OP_RX:
+                case QFVM_OP_RY:
+                case QFVM_OP_RZ:
+                case QFVM_OP_CP:
+                case QFVM_OP_RZZ: d.param = -op.param; break;
+                case QFVM_OP_X: case QFVM_OP_Y: case QFVM_OP_Z: case QFVM_OP_H:
+                case QFVM_OP_CX: case QFVM_OP_CZ: case QFVM_OP_CCX: break;
+                default: throw QfvmError(QFVM_ERR_VALUE, "qfvm_adjoint takes unitary ops only");
+            }
+        };
+        auto apply_dagger = [&](const qfvm_op& op) {
+            qfvm_op d{};
+            dagger_of(op, d);
+            qfvm_program one{&d, 1, idx.data(), (int64_t)idx.size(), mat.data(), (int64_t)mat.size()};
+            apply_gate(h, &one, d);
+        };
+        std::vector<cd> phi = psi;
+        for (int64_t i = prog->n_ops - 1; i >= begin; i--) {
+            const qfvm_op& op = prog->ops[i];
+            h->data.swap(phi);
+            apply_dagger(op);
+            h->data.swap(phi);
+            const int64_t g = slot_of[(size_t)i];
+            if (g >= 0) {
+                std::vector<int> q = op_list(prog, op.q_off, op.n_qubits);
+                const int nc = (op.code == QFVM_OP_GENERAL) ? op.n_ctrl : 0;
+                if ((int)q.size() != nc + 1) throw QfvmError(QFVM_ERR_VALUE, "differentiated ops have one target");
+                const int target = q[(size_t)nc];
+                cd u[4];
+                const double th = op.param;
+                switch (op.code) {
+                    case QFVM_OP_RX:
+                        u[0] = std::cos(th / 2); u[1] = -kI * std::sin(th / 2);
+                        u[2] = -kI * std::sin(th / 2); u[3] = std::cos(th / 2);
+                        break;
+                    case QFVM_OP_RY:
+                        u[0] = std::cos(th / 2); u[1] = -std::sin(th / 2);
+                        u[2] = std::sin(th / 2); u[3] = std::cos(th / 2);
+                        break;
+                    case QFVM_OP_RZ:
+                        u[0] = std::exp(-kI * th / 2.); u[1] = 0.0; u[2] = 0.0; u[3] = std::exp(kI * th / 2.);
+                        break;
+                    case QFVM_OP_GENERAL: {
+                        std::vector<cd> m = op_matrix(prog, op, 2);
+                        for (int e = 0; e < 4; e++) u[e] = m[(size_t)e];
+                        break;
+                    }
+                    default: throw QfvmError(QFVM_ERR_VALUE, "differentiated ops are rotations");
+                }
+                cd p[4];
+                switch (grad_gen[g]) {
+                    case 'X': p[0] = 0.0; p[1] = 1.0; p[2] = 1.0; p[3] = 0.0; break;
+                    case 'Y': p[0] = 0.0; p[1] = -kI; p[2] = kI; p[3] = 0.0; break;
+                    case 'Z': p[0] = 1.0; p[1] = 0.0; p[2] = 0.0; p[3] = -1.0; break;
+                    default: throw QfvmError(QFVM_ERR_VALUE, "gradient generators are X, Y or Z");
+                }
+                cd du[4];
+                for (int r = 0; r < 2; r++)
+                    for (int c = 0; c < 2; c++)
+                        du[2 * r + c] = -0.5 * kI * (p[2 * r] * u[c] + p[2 * r + 1] * u[2 + c]);
+                h->data = phi;
+                h->gate_general(0, {target}, du);
+                uint64_t cm = 0;
+                for (int c = 0; c < nc; c++) cm |= 1ull << q[(size_t)c];
+                double acc = 0.0;
+                const cd* mu = h->data.data();
+                const cd* l = lam.data();
+#pragma omp parallel for reduction(+ : acc)
+                for (int64_t j = 0; j < (int64_t)h->size; j++)
+                    if (((uint64_t)j & cm) == cm) acc += std::real(std::conj(l[j]) * mu[j]);
+                grads[g] = 2.0 * acc;
+            }
+            h->data.swap(lam);
+            apply_dagger(op);
+            h->data.swap(lam);
+        }
+        h->data = phi;
+    });
+}
+
+/*
  * Oracle-only: the reference's own sampler semantics (measure_samples, statevector.hpp:1380-1412):
  * for each uniform, the first index whose sequential running sum of |amp|^2 exceeds it, 2^n if none.
  * Serial over shots, as the reference's orphaned `omp for` is.

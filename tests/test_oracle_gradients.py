@@ -88,3 +88,51 @@ def test_hamiltonian_apply_matches_dense_matrices():
 
     want = sum(c * (dense.pauli_matrix_le(s, p, n) @ psi) for (s, p), c in zip(terms, coeffs))
     assert np.abs(og.hamiltonian_apply(psi, n, terms, coeffs) - want).max() < 1e-12
+
+
+# ------------------------------------------------------------------ the C++ oracle's grad_adjoint
+@pytest.mark.parametrize("theta", [-2.3, 0.0, 0.9])
+def test_cpp_adjoint_closed_forms(theta):
+    """qfvm_adjoint of the C++ oracle (oracle/qfvm_cpu.cpp) on the one-rotation closed forms."""
+    from oracle.cpu import CpuState
+
+    cases = [
+        (RY, (), "Z", np.cos(theta), -np.sin(theta), "Y"),
+        (RX, (), "Y", -np.sin(theta), -np.cos(theta), "X"),
+        (RZ, (H,), "X", np.cos(theta), -np.sin(theta), "Z"),
+    ]
+    for code, pre, obs, e_want, g_want, gen in cases:
+        prog = one_rotation(code, pre)([theta])
+        s = CpuState(1)
+        g, e = s.adjoint(prog, [len(pre)], gen, [(obs, [0])], [1.0])
+        assert abs(e - e_want) < 1e-14
+        assert abs(g[0] - g_want) < 1e-14
+
+
+@pytest.mark.parametrize("seed,max_ctrl", [(0, 0), (1, 1), (2, 2), (3, 2)])
+def test_cpp_adjoint_matches_dense_oracle(seed, max_ctrl):
+    """The C++ restatement against the dense NumPy one (itself pinned above by closed forms, the
+    exact shift rule and finite differences) on random rotation programs with controls, complex
+    coefficients and a random input state; the state left behind is the one before the first
+    differentiated op."""
+    from oracle.cpu import CpuState
+    from tests.conftest import rand_state
+
+    rng = np.random.Generator(np.random.PCG64(1200 + seed))
+    n = 6
+    make, thetas, ops, gens = rotation_program(n, 40, rng, max_ctrl=max_ctrl)
+    terms, coeffs = random_pauli_terms(n, 5, rng)
+    coeffs = [complex(c) for c in coeffs]
+    psi0 = rand_state(n, rng)
+    prog = make(thetas)
+    g_want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs, psi0=psi0)
+    s = CpuState(n)
+    s.load(psi0)
+    g, e = s.adjoint(prog, ops, gens, terms, coeffs)
+    assert abs(e - e_want) < 1e-12
+    assert np.abs(g - g_want).max() < 1e-12
+    from oracle import dense
+
+    r = dense.DenseRunner(prog, n, psi0)
+    r.exec(0, min(ops))
+    assert np.abs(s.store() - r.psi).max() < 1e-12
