@@ -213,13 +213,13 @@ int qfvm_probabilities(qfvm_handle h, double* out, uint64_t n);
 int qfvm_marginals(qfvm_handle h, const int32_t* qubits, int32_t n_qubits, double* out);
 
 /*
- * Sample basis-state indices, one per uniform u in [0,1). Replaces the per-shot CDF scan of
- * measure_samples (statevector.hpp:1380-1412): the result is the basis index whose cumulative
- * probability interval contains u, found by descending a pairwise sum tree of p_i = |amp_i|^2
- * (node = left + right, p_i = fma(re, re, im*im) in double; go left while u < base + left,
- * never into a zero-sum child). u >= the root sum gives 2^n, whose measured bits are all zero,
- * as in the reference. The index depends only on the state and u, so any implementation of
- * the same tree reproduces it bit for bit.
+ * Sample basis-state indices, one per uniform u in [0,1), with measure_samples' own rule
+ * (statevector.hpp:1380-1412, inner loop :1390-1396): the first index k whose running sum
+ * S_k = S_{k-1} + p_k (one double rounding per add, in index order, p_k = fma(re, re, im*im) as
+ * the reference's real(d * conj(d)) compiles under -mfma -ffast-math) satisfies u < S_k; 2^n when
+ * none does (measured bits all zero, as in the reference). Bit-exact with that sequential scan for
+ * every state and u, but the sums are evaluated once per call, in parallel (exact integer steps per
+ * binade of S), and each shot is a binary search: O(2^n + shots log 2^n) instead of O(shots 2^n).
  */
 int qfvm_sample(qfvm_handle h, const double* uniforms, int64_t shots, uint64_t* out_index);
 

@@ -229,8 +229,9 @@ class CpuState:
         return v.value
 
 
-def sample_tree_on(psi: np.ndarray, uniforms) -> np.ndarray:
-    """Tree-sampler indices for an explicit host state (e.g. a state read back from the GPU)."""
+def sample_on(psi: np.ndarray, uniforms) -> np.ndarray:
+    """measure_samples' indices (qfvm_sample: sequential running sum, binary search per shot) for an
+    explicit host state, e.g. a state read back from the GPU (complex64 states widen exactly)."""
     n = int(np.log2(psi.size))
     s = CpuState(n)
     s.load(psi)

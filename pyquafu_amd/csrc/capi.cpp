@@ -992,7 +992,7 @@ int qfvm_sample(qfvm_handle h, const double* uniforms, int64_t shots, uint64_t* 
         State* s = need(h);
         if (shots < 0) throw Error(QFVM_ERR_VALUE, "negative shot count");
         if (shots > 0 && (!uniforms || !out_index)) throw Error(QFVM_ERR_VALUE, "null buffer");
-        sample_tree(*s, uniforms, shots, out_index);
+        sample_sequential_rule(*s, uniforms, shots, out_index);
     });
 }
 

@@ -245,7 +245,8 @@ double reduce_pauli(State& s, uint64_t flip_mask, uint64_t z_mask, int flip_q, i
 void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double>& probs);
 void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale);
 void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b);
-void sample_tree(State& s, const double* host_u, int64_t shots, uint64_t* host_out);
+// measure_samples' rule: per uniform, the first index whose sequential running sum of |amp|^2 exceeds it
+void sample_sequential_rule(State& s, const double* host_u, int64_t shots, uint64_t* host_out);
 // Re <psi|H|psi>; when `out` is not null, H psi is written there (a buffer shaped like the state).
 // With dev_result, the value is left in that device double instead (no sync; returns 0).
 double apply_hamiltonian(State& s, const Hamiltonian& H, void* out, double* dev_result = nullptr);

@@ -8,7 +8,7 @@ import pytest
 
 import pyquafu_amd as pq
 from oracle import dense
-from oracle.cpu import sample_tree_on
+from oracle.cpu import sample_on
 from pyquafu_amd import _native
 from pyquafu_amd.algorithms.hamiltonian import Hamiltonian, PauliOp
 from pyquafu_amd.elements import (AmplitudeDamping, BitFlip, Decoherence, Dephasing, Depolarizing,
@@ -114,7 +114,7 @@ def test_kraus_branch_boundaries_on_device():
 
 def oracle_noisy_run(qc, shots, seed, hamiltonian=None, psi=None):
     """The trajectory loop of NoiseSVSimulator.run restated on the oracle: per shot the uniforms
-    for the channels, then one for the sample (tree sampler on the oracle's state)."""
+    for the channels, then one for the sample (the sequential-rule sampler on the oracle's state)."""
     prog = NoiseSVSimulator.trajectory_program(qc)
     n = qc.num
     rng = np.random.default_rng(seed)
@@ -124,7 +124,7 @@ def oracle_noisy_run(qc, shots, seed, hamiltonian=None, psi=None):
     for _ in range(shots):
         u = rng.random(prog.n_random + 1)
         out, _ = dense.run_dense(prog, n, psi=psi, uniforms=u[:-1])
-        idx = int(sample_tree_on(out, u[-1:])[0])
+        idx = int(sample_on(out, u[-1:])[0])
         bits = ["0"] * len(qc.measures)
         for q, c in qc.measures.items():
             bits[c] = str((idx >> q) & 1) if idx < (1 << n) else "0"

@@ -6,7 +6,7 @@ import pytest
 
 import pyquafu_amd as pq
 from oracle import dense
-from oracle.cpu import CpuState, sample_tree_on
+from oracle.cpu import CpuState, sample_on
 from pyquafu_amd import _native, workloads
 from pyquafu_amd.simulators.program import compile_circuit
 from tests.conftest import rand_state
@@ -89,11 +89,62 @@ def test_sampler_bit_exact(n, dtype):
     u[:4] = [0.0, 1.0 - 2**-53, 1.0, 2.0]
     got = dev.sample(u)
     on_gpu_state = dev.store()  # c64 states come back widened exactly
-    want = sample_tree_on(on_gpu_state, u)
+    want = sample_on(on_gpu_state, u)
     assert np.array_equal(got, want)
     nz = np.abs(on_gpu_state) > 0
     drawn = got[got < (1 << n)].astype(np.int64)
     assert nz[drawn].all()
+
+
+def crafted_state(kind: str, n: int, rng) -> np.ndarray:
+    """States that stress the exact evaluation of the sequential sum: ties (|amp|^2 exactly half an
+    ulp of the running sum), many binade crossings spread through the array, long zero runs,
+    unnormalised and tiny magnitudes."""
+    size = 1 << n
+    if kind == "random":
+        return rand_state(n, rng)
+    if kind == "ties":
+        t = np.full(size, 2.0**-27, complex)  # p = 2^-54 while the sum is in [0.5, 1)
+        t[0] = np.sqrt(0.5)
+        t[rng.random(size) < 0.3] *= np.sqrt(3.0)
+        t[rng.random(size) < 0.1] = 0.0
+        return t
+    if kind == "growing":
+        # |amp|^2 grows by 2^40 over the array: the sum crosses ~40 binades at scattered places
+        mag = 2.0 ** (-60 + 40 * np.arange(size) / size) * (0.5 + rng.random(size))
+        return np.sqrt(mag) * np.exp(2j * np.pi * rng.random(size))
+    if kind == "zeros":
+        z = rand_state(n, rng)
+        z[: size // 3] = 0.0
+        z[rng.random(size) < 0.5] = 0.0
+        z[-size // 5:] = 0.0
+        return z
+    if kind == "unnormalised":
+        return rand_state(n, rng) * 37.5
+    if kind == "tiny":
+        return rand_state(n, rng) * 1e-150
+    raise KeyError(kind)
+
+
+@pytest.mark.parametrize("kind", ["random", "ties", "growing", "zeros", "unnormalised", "tiny"])
+@pytest.mark.parametrize("n", [9, 16, 22])
+def test_sampler_sequential_rule_states(kind, n):
+    """The GPU sampler against measure_samples' rule (the oracle's qfvm_sample, itself pinned
+    against the reference's per-shot scan) on crafted states, with uniforms spread over the range
+    and placed exactly on, just below and just above cumulative boundaries."""
+    rng = np.random.Generator(np.random.PCG64(500 + n + sum(map(ord, kind))))
+    psi = crafted_state(kind, n, rng)
+    dev = gpu_state(n, "c128")
+    dev.load(psi)
+    cdf = np.cumsum(np.abs(psi) ** 2)
+    total = float(cdf[-1])
+    pick = rng.integers(0, psi.size, size=3000)
+    u = np.concatenate([rng.random(20000) * total * 1.001, [0.0, total, 2.0 * total], cdf[pick],
+                        np.nextafter(cdf[pick], 0.0), np.nextafter(cdf[pick], np.inf)])
+    got = dev.sample(u)
+    want = sample_on(psi, u)
+    print(f"[sampler {kind} n={n}] {int((got != want).sum())} mismatches of {u.size}", flush=True)
+    assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize("seed", [21, 22, 23, 24, 25])
@@ -131,8 +182,12 @@ def test_parity_workload_20q():
     assert err < 1e-12
     u = np.random.Generator(np.random.PCG64(20250001)).random(100000)
     idx = dev.sample(u)
-    assert np.array_equal(idx, sample_tree_on(got, u))
-    assert np.array_equal(idx, cpu.sample(u))
+    # measure_samples' rule on the GPU's state: all 10^5 shots through the oracle's sequential
+    # pass + search, and the first 2,000 through the reference's own per-shot scan
+    held = CpuState(n)
+    held.load(got)
+    assert np.array_equal(idx, held.sample(u))
+    assert np.array_equal(idx[:2000], held.sample_sequential(u[:2000]))
 
 
 # ------------------------------------------------------------------ reference known answers

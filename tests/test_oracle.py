@@ -6,8 +6,8 @@
   every op code, controls anywhere and dense k-target matrices;
 * expect_pauli against dense Pauli matrices, which are themselves pinned by the reference's
   Hamiltonian fixtures (tests/golden/reference_fixtures.npz);
-* the sum-tree sampler against the reference's sequential CDF rule, exact dyadic cases and
-  deterministic states;
+* the sampler (one sequential pass + binary search) against the reference's per-shot scan,
+  exact dyadic cases and deterministic states;
 * measure / reset / cif against the dense runner with the same uniforms.
 """
 
@@ -187,26 +187,46 @@ def test_expect_against_reference_hamiltonian_fixtures(rng):
 
 
 # ------------------------------------------------------------------ sampler
-def test_tree_sampler_matches_sequential_rule(rng):
+def sampler_states(rng):
+    """Named states for the sampler checks: random, dyadic with zeros, tie-heavy (|amp|^2 exactly
+    half an ulp of the running sum), unnormalised, leading / trailing zeros, tiny probabilities."""
     n = 10
-    psi = rand_state(n, rng)
+    out = {"random": rand_state(n, rng)}
+    psi = np.array([0, 0.5, 0, 0.5j, 0.25 + 0.25j, 0, 0.25 - 0.25j, -0.5], complex)
+    out["dyadic"] = psi
+    t = np.full(1 << n, 2.0**-27, complex)  # p = 2^-54: a tie once the sum reaches [0.5, 1)
+    t[0] = np.sqrt(0.5)
+    t[rng.random(1 << n) < 0.3] *= np.sqrt(3.0)
+    out["ties"] = t
+    out["unnormalised"] = rand_state(n, rng) * 37.5
+    z = rand_state(n, rng)
+    z[:700] = 0.0
+    z[1000:] = 0.0
+    out["zeros"] = z
+    out["tiny"] = rand_state(n, rng) * 1e-150
+    return out
+
+
+@pytest.mark.parametrize("name", ["random", "dyadic", "ties", "unnormalised", "zeros", "tiny"])
+def test_sampler_is_the_sequential_rule(name):
+    """qfvm_sample (one sequential pass + binary search) against qfvm_sample_sequential, which is
+    the reference's per-shot scan compiled with the reference's flags: identical for every u,
+    including u on a boundary, 0, >= the total, and values within an ulp of boundaries."""
+    rng = np.random.Generator(np.random.PCG64(sum(map(ord, name))))
+    psi = sampler_states(rng)[name]
+    n = int(np.log2(psi.size))
     s = CpuState(n)
     s.load(psi)
-    u = rng.random(100000)
-    tree = s.sample(u)
-    seq = s.sample_sequential(u)
-    cdf = np.cumsum(np.abs(psi) ** 2)
-    mism = np.nonzero(tree != seq)[0]
-    # any disagreement must sit within rounding of a cumulative boundary
-    for k in mism:
-        assert np.min(np.abs(cdf - u[k])) < 1e-12
-    assert len(mism) <= 2
+    p = np.abs(psi) ** 2
+    cdf = np.cumsum(p)
+    u = np.concatenate([rng.random(4000) * cdf[-1] * 1.01, [0.0, cdf[-1], 2.0 * cdf[-1] + 1.0],
+                        cdf[::7], np.nextafter(cdf[::5], 0.0), np.nextafter(cdf[::11], np.inf)])
+    assert np.array_equal(s.sample(u), s.sample_sequential(u))
 
 
-def test_tree_sampler_exact_dyadic():
-    # probabilities that are exact binary fractions: every partial sum is exact, so the descent must
-    # give exactly "first index with u < CDF", skipping zero-probability states
-    # amplitudes whose squared moduli are exact: p = [0, 1/4, 0, 1/4, 1/8, 0, 1/8, 1/4]
+def test_sampler_exact_dyadic():
+    # probabilities that are exact binary fractions: every partial sum is exact, so the rule is
+    # exactly "first index with u < CDF", skipping zero-probability states
     psi = np.array([0, 0.5, 0, 0.5j, 0.25 + 0.25j, 0, 0.25 - 0.25j, -0.5], complex)
     p = np.abs(psi) ** 2
     assert p.sum() == 1.0
@@ -220,7 +240,7 @@ def test_tree_sampler_exact_dyadic():
     assert set(got.tolist()) == {1, 3, 4, 6, 7}
 
 
-def test_tree_sampler_special_states():
+def test_sampler_special_states():
     s = CpuState(4)
     s.init_basis(11)
     u = np.array([0.0, 0.3, 0.999999, 1.0, 1.5])
