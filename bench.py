@@ -429,6 +429,24 @@ def main(argv=None) -> int:
     achieved = per_launch_bytes / per_launch_s / 1e9
     norm_err = abs(dev.norm2() - 1.0)
 
+    # expect_statevec of the periodic TFIM (n ZZ + n X terms) on the resident final state: all terms
+    # in a few LDS-tiled passes, one host sync
+    tfim = [("ZZ", [i, (i + 1) % n]) for i in range(n)] + [("X", [i]) for i in range(n)]
+    dev.expect(tfim)
+    dev.synchronize()
+    dev.profile(reset=True)
+    t0 = time.perf_counter()
+    tfim_vals = dev.expect(tfim)
+    t_tfim = time.perf_counter() - t0
+    tfim_prof = dev.profile(reset=True)
+    expect_tfim = {
+        "terms": len(tfim),
+        "ms": round(t_tfim * 1e3, 3),
+        "launches": {k: v["launches"] for k, v in tfim_prof.items()},
+        "state_reads": int(sum(v["launches"] for k, v in tfim_prof.items() if k in ("expect_tile", "pauli_expect"))),
+        "energy_h0.7": float(sum(tfim_vals[:n]) + 0.7 * sum(tfim_vals[n:])),
+    }
+
     ge = gate_equiv_bytes(prog, n, amp)
     out = {
         "metric": METRIC,
@@ -467,6 +485,7 @@ def main(argv=None) -> int:
         },
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in prof.items()},
         "norm_error": norm_err,
+        "expect_tfim": expect_tfim,
     }
     dev.close()
     if not args.no_adjoint:

@@ -156,8 +156,10 @@ int qfvm_run(qfvm_handle h, const qfvm_program* prog, const double* uniforms, in
 
 /*
  * Pauli-string expectations <psi|P_t|psi>, unscaled by coefficients. Replaces expect_statevec
- * (qfvm.cpp:246-261) -> expect_pauli (statevector.hpp:1250-1305). Term t uses
- * chars[off[t] .. off[t]+len[t]) ('I','X','Y','Z') on qubits pos[off[t] ..].
+ * (qfvm.cpp:246-261) -> expect_pauli (statevector.hpp:1250-1305, its formula per term). Term t uses
+ * chars[off[t] .. off[t]+len[t]) ('I','X','Y','Z') on qubits pos[off[t] ..]. All terms are taken
+ * in few passes over the state (the terms whose X / Y qubits fit one LDS tile share a pass) with
+ * one host synchronisation per call, instead of the reference's pass per term.
  */
 int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
                        const int32_t* len, int64_t n_terms, double* out);

@@ -811,25 +811,51 @@ int qfvm_run(qfvm_handle h, const qfvm_program* prog, const double* uniforms, in
     });
 }
 
+// expect_pauli's masks (statevector.hpp:1252-1279): X and Y flip, Y and Z weigh by parity, the
+// number of Y letters sets the phase. The reference adds the bits (`+= 1 << q`), so a qubit named
+// twice carries into the next bit, possibly past the last qubit, and its scan then reads outside the
+// state; such a term is rejected here.
+static PauliHost parse_pauli(const State& s, const char* chars, const int32_t* pos, int64_t off, int32_t len,
+                             int slot) {
+    PauliHost t{};
+    t.slot = slot;
+    uint64_t seen = 0;
+    for (int32_t k = 0; k < len; k++) {
+        const int q = pos[off + k];
+        if (q < 0 || q >= s.n) throw Error(QFVM_ERR_INDEX, "pauli qubit out of range");
+        const char c = chars[off + k];
+        if (c != 'X' && c != 'Y' && c != 'Z') continue;
+        if ((seen >> q) & 1) throw Error(QFVM_ERR_VALUE, "repeated qubit in a Pauli term");
+        seen |= 1ull << q;
+        if (c == 'X' || c == 'Y') {
+            t.flip += 1ull << q;
+            t.flip_q = std::max(t.flip_q, q);
+        }
+        if (c == 'Y') t.y++;
+        if (c == 'Y' || c == 'Z') t.z += 1ull << q;
+    }
+    return t;
+}
+
 int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off, const int32_t* len,
                        int64_t n_terms, double* out) {
     return guarded([&] {
         State* s = need(h);
-        for (int64_t t = 0; t < n_terms; t++) {
-            uint64_t flip = 0, z = 0;
-            int flip_q = 0, y_num = 0;
-            for (int32_t k = 0; k < len[t]; k++) {
-                const int q = pos[off[t] + k];
-                if (q < 0 || q >= s->n) throw Error(QFVM_ERR_INDEX, "pauli qubit out of range");
-                switch (chars[off[t] + k]) {  // statevector.hpp:1257-1279 (sums, as the reference)
-                    case 'X': flip += 1ull << q; flip_q = std::max(flip_q, q); break;
-                    case 'Y': flip += 1ull << q; flip_q = std::max(flip_q, q); y_num++; z += 1ull << q; break;
-                    case 'Z': z += 1ull << q; break;
-                    default: break;
-                }
-            }
-            out[t] = reduce_pauli(*s, flip, z, flip_q, y_num);
+        if (n_terms < 0) throw Error(QFVM_ERR_VALUE, "negative term count");
+        if (n_terms == 0) return;
+        if (!chars || !pos || !off || !len || !out) throw Error(QFVM_ERR_VALUE, "null term array");
+        std::vector<PauliHost> terms((size_t)n_terms);
+        for (int64_t t = 0; t < n_terms; t++) terms[(size_t)t] = parse_pauli(*s, chars, pos, off[t], len[t], (int)t);
+        double* out_dev = reinterpret_cast<double*>(s->ensure_aux((size_t)n_terms * sizeof(double)));
+        double* partials = reinterpret_cast<double*>(s->ensure_scratch((EXPECT_PARTIALS + RED_PARTIALS) * sizeof(double)));
+        for (int64_t t0 = 0; t0 < n_terms; t0 += EXPECT_BATCH) {
+            const int cnt = (int)std::min<int64_t>(EXPECT_BATCH, n_terms - t0);
+            queue_expectations(*s, terms.data() + t0, cnt, out_dev, partials, partials + EXPECT_PARTIALS);
         }
+        double* hbuf = s->ensure_pinned((size_t)n_terms * sizeof(double));
+        QFVM_HIP(hipMemcpyAsync(hbuf, out_dev, (size_t)n_terms * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        QFVM_HIP(hipStreamSynchronize(s->stream));
+        std::memcpy(out, hbuf, (size_t)n_terms * sizeof(double));
     });
 }
 

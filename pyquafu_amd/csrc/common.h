@@ -241,7 +241,20 @@ void launch_h2d_convert(State& s, const double* host, uint64_t n);  // host c128
 void launch_d2h_convert(State& s, double* host, uint64_t n);
 double reduce_norm2(State& s);
 void reduce_probabilities(State& s, double* host_out);
-double reduce_pauli(State& s, uint64_t flip_mask, uint64_t z_mask, int flip_q, int y_num);
+// Pauli-string expectations (expect_pauli's formula) of up to EXPECT_BATCH terms, queued on the
+// stream: term t's value goes to out_dev[terms[t].slot]. Scratch is the caller's: `partials` holds at
+// least EXPECT_PARTIALS doubles, `fallback` RED_PARTIALS. With account = false nothing is counted
+// either, and the call allocates nothing (no device, pinned or host heap memory).
+struct PauliHost {
+    uint64_t flip, z;  // X / Y qubits; Z / Y qubits
+    int flip_q, y;     // highest X / Y qubit; number of Y letters
+    int slot;
+};
+constexpr int EXPECT_BATCH = 256;
+constexpr size_t EXPECT_PARTIALS = 32 * 2048;
+constexpr size_t RED_PARTIALS = 2048 + 1;
+void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* out_dev, double* partials,
+                        double* fallback, bool account = true);
 void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double>& probs);
 void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale);
 void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b);

@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <optional>
 
 #include "common.h"
 
@@ -153,6 +155,117 @@ __global__ void k_swap_states(V2<R>* __restrict__ psi, uint64_t size, uint64_t q
 }
 
 // ------------------------------------------------------------------------------------------------
+// Many Pauli-string expectations in few passes (expect_statevec, qfvm.cpp:246-261, term by term
+// through expect_pauli, statevector.hpp:1250-1305). A pass streams the state through LDS in tiles
+// of 2^b amplitudes whose qubits are the L lowest plus up to b - L chosen higher ones; every term
+// whose X / Y qubits all lie in the tile is evaluated from LDS: its pairs (i0, i1 = i0 ^ flip, i0
+// with the highest flip bit clear) are both in the tile, and its Z / Y mask only needs the global
+// index. Terms are packed first-fit into passes, so a Hamiltonian of Z strings and single-qubit X
+// terms on n qubits needs about (n - L) / (b - L) passes instead of one pass per term. Workgroups
+// walk a fixed set of tiles and add each tile's block sum to a per-term accumulator in tile order;
+// the per-workgroup accumulators are then added in index order (deterministic).
+// Algorithmic bytes per pass: one read of the state (s 2^n).
+constexpr int PT_MAX = 32;        // terms per launch (kernel argument)
+constexpr unsigned PT_GRID = 2048;
+static_assert(EXPECT_PARTIALS >= (size_t)PT_MAX * PT_GRID, "expectation partials");
+static_assert(RED_PARTIALS >= RED_GRID + 1, "reduction partials");
+
+struct PTerm {
+    uint64_t z;     // Z / Y mask, global bits
+    uint32_t f;     // flip mask in tile-local bits; 0 for a term of Z and I only
+    int32_t fq;     // tile-local position of the highest flip bit
+    int32_t y;      // number of Y letters
+    int32_t slot;   // output index
+};
+
+struct PPass {
+    int b, L, nterms;
+    int q[16];  // tile qubits, ascending; q[k] = k for k < L
+    PTerm t[PT_MAX];
+};
+
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restrict__ psi, const PPass pp,
+                                                           uint64_t tiles, double* __restrict__ partials) {
+    extern __shared__ unsigned char lds_raw[];
+    const int b = pp.b, L = pp.L, T = 1 << b;
+    V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
+    uint64_t* hi = reinterpret_cast<uint64_t*>(tile + T);  // global offsets of the tile's high bits
+    double* red = reinterpret_cast<double*>(hi + (1 << (b - L)));
+    double* acc = red + 8 * (RED_BLOCK / 64);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t lowmask = (1ull << L) - 1;
+    for (int i = tid; i < (1 << (b - L)); i += RED_BLOCK) {
+        uint64_t g = 0;
+        for (int k = L; k < b; k++)
+            if ((i >> (k - L)) & 1) g |= 1ull << pp.q[k];
+        hi[i] = g;
+    }
+    if (tid < PT_MAX) acc[tid] = 0.0;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        uint64_t base = t;
+        for (int k = 0; k < b; k++) {
+            const uint64_t low = (1ull << pp.q[k]) - 1;
+            base = (base & low) | ((base & ~low) << 1);
+        }
+        __syncthreads();  // the previous tile is no longer read; hi is built
+        for (int e = tid; e < T; e += RED_BLOCK) tile[e] = psi[base | ((uint64_t)e & lowmask) | hi[e >> L]];
+        __syncthreads();
+        for (int c0 = 0; c0 < pp.nterms; c0 += 8) {
+            double a[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = 0.0;
+            for (int e = tid; e < T; e += RED_BLOCK) {
+                const uint64_t G = base | ((uint64_t)e & lowmask) | hi[e >> L];
+                const V2<R> x = tile[e];
+                const double p = (double)x.x * x.x + (double)x.y * x.y;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    if (c0 + k < pp.nterms) {
+                        const PTerm& tm = pp.t[c0 + k];
+                        const int zpar = __popcll(G & tm.z) & 1;
+                        if (tm.f == 0) {
+                            a[k] += zpar ? -p : p;
+                        } else if (!((e >> tm.fq) & 1)) {
+                            const V2<R> w = tile[e ^ (int)tm.f];
+                            const double re = (double)x.x * w.x + (double)x.y * w.y;  // Re(x conj(w))
+                            const double im = (double)x.y * w.x - (double)x.x * w.y;  // Im(x conj(w))
+                            const int kk = (2 * zpar + tm.y) & 3;  // phase i^kk (PHASE_YZ = 1, i, -1, -i)
+                            const double v = kk == 0 ? re : kk == 1 ? -im : kk == 2 ? -re : im;
+                            a[k] += 2.0 * v;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const double v = wave_sum(a[k]);
+                if (lane == 0) red[k * (RED_BLOCK / 64) + wave] = v;
+            }
+            __syncthreads();
+            if (tid < 8 && c0 + tid < pp.nterms) {
+                double v = 0.0;
+                for (int w = 0; w < (int)(RED_BLOCK / 64); w++) v += red[tid * (RED_BLOCK / 64) + w];
+                acc[c0 + tid] += v;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (tid < pp.nterms) partials[(uint64_t)tid * gridDim.x + blockIdx.x] = acc[tid];
+}
+
+// per-term sums of a launch's partials into out[term.slot]
+__global__ void __launch_bounds__(RED_BLOCK) k_term_sums(const double* __restrict__ partials, unsigned cols,
+                                                         const PPass pp, double* __restrict__ out) {
+    const double* col = partials + (uint64_t)blockIdx.x * cols;
+    double a = 0.0;
+    for (unsigned k = threadIdx.x; k < cols; k += blockDim.x) a += col[k];
+    a = block_sum(a);
+    if (threadIdx.x == 0) out[pp.t[blockIdx.x].slot] = a;
+}
+
+// ------------------------------------------------------------------------------------------------
 static double finish_sum(State& s, double* partial, unsigned n) {
     double* out = partial + n;
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s.stream, partial, (int)n, out);
@@ -176,19 +289,107 @@ double reduce_norm2(State& s) {
     return finish_sum(s, partial, grid);
 }
 
-double reduce_pauli(State& s, uint64_t flip, uint64_t zmask, int flip_q, int y_num) {
-    double* partial = reinterpret_cast<double*>(s.ensure_scratch((RED_GRID + 1) * sizeof(double)));
-    const uint64_t work = flip ? (s.size >> 1) : s.size;
+// Queues k_pauli for one term whose X / Y qubits do not fit one tile, leaving the value in *out.
+static void queue_pauli_single(State& s, const PauliHost& t, double* partial, double* out, bool account) {
+    const uint64_t work = t.flip ? (s.size >> 1) : s.size;
     const unsigned grid = std::min<unsigned>(RED_GRID, grid_for(work, RED_BLOCK));
-    KernelScope ks(s, "pauli_expect", (double)s.size * s.amp_bytes());
+    std::optional<KernelScope> ks;
+    if (account) ks.emplace(s, "pauli_expect", (double)s.size * s.amp_bytes());
     if (s.dtype == QFVM_C128)
         hipLaunchKernelGGL(k_pauli<double>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
-                           reinterpret_cast<const double2*>(s.psi), work, flip, zmask, flip_q, y_num, partial);
+                           reinterpret_cast<const double2*>(s.psi), work, t.flip, t.z, t.flip_q, t.y, partial);
     else
         hipLaunchKernelGGL(k_pauli<float>, dim3(grid), dim3(RED_BLOCK), 0, s.stream,
-                           reinterpret_cast<const float2*>(s.psi), work, flip, zmask, flip_q, y_num, partial);
+                           reinterpret_cast<const float2*>(s.psi), work, t.flip, t.z, t.flip_q, t.y, partial);
     QFVM_LAUNCHED();
-    return finish_sum(s, partial, grid);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RED_BLOCK), 0, s.stream, partial, (int)grid, out);
+    QFVM_LAUNCHED();
+}
+
+void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* out_dev, double* partials,
+                        double* fallback, bool account) {
+    if (n_terms > EXPECT_BATCH) throw Error(QFVM_ERR_VALUE, "too many terms in one expectation batch");
+    const int n = s.n;
+    const int b = std::min(n, s.dtype == QFVM_C128 ? 11 : 12);
+    const int L = b < n ? std::min(5, b - 1) : b;
+    const uint64_t lowmask = (1ull << L) - 1;
+    const uint64_t tiles = 1ull << (n - b);
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, PT_GRID);
+    bool done[EXPECT_BATCH];
+    int order[EXPECT_BATCH];
+    for (int i = 0; i < n_terms; i++) done[i] = false;
+    int left = n_terms;
+    while (left > 0) {
+        uint64_t Qh = 0;
+        int cnt = 0;
+        for (int i = 0; i < n_terms; i++) {
+            if (done[i]) continue;
+            const uint64_t hb = terms[i].flip & ~lowmask;
+            if (__builtin_popcountll(Qh | hb) <= b - L) {
+                Qh |= hb;
+                order[cnt++] = i;
+                done[i] = true;
+            }
+        }
+        if (cnt == 0) {  // the first remaining term cannot share a tile
+            for (int i = 0; i < n_terms; i++)
+                if (!done[i]) {
+                    queue_pauli_single(s, terms[i], fallback, out_dev + terms[i].slot, account);
+                    done[i] = true;
+                    left--;
+                    break;
+                }
+            continue;
+        }
+        left -= cnt;
+        for (int q = L; q < n && __builtin_popcountll(Qh) < b - L; q++) Qh |= 1ull << q;
+        PPass pp{};
+        pp.b = b;
+        pp.L = L;
+        int local_of[64];
+        for (int k = 0; k < 64; k++) local_of[k] = -1;
+        for (int k = 0; k < L; k++) {
+            pp.q[k] = k;
+            local_of[k] = k;
+        }
+        int kq = L;
+        for (int q = L; q < n; q++)
+            if ((Qh >> q) & 1) {
+                pp.q[kq] = q;
+                local_of[q] = kq++;
+            }
+        for (int c0 = 0; c0 < cnt; c0 += PT_MAX) {
+            pp.nterms = std::min(PT_MAX, cnt - c0);
+            for (int k = 0; k < pp.nterms; k++) {
+                const PauliHost& t = terms[order[c0 + k]];
+                PTerm& d = pp.t[k];
+                d.z = t.z;
+                d.f = 0;
+                d.fq = 0;
+                for (int q = 0; q < n; q++)
+                    if ((t.flip >> q) & 1) {
+                        d.f |= 1u << local_of[q];
+                        d.fq = local_of[q];
+                    }
+                d.y = t.y;
+                d.slot = t.slot;
+            }
+            const size_t lds = ((size_t)1 << b) * s.amp_bytes() + ((size_t)1 << (b - L)) * sizeof(uint64_t) +
+                               (8 * (RED_BLOCK / 64) + PT_MAX) * sizeof(double);
+            std::optional<KernelScope> ks;
+            if (account) ks.emplace(s, "expect_tile", (double)s.size * s.amp_bytes());
+            if (s.dtype == QFVM_C128)
+                hipLaunchKernelGGL(k_expect_tile<double>, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+                                   reinterpret_cast<const double2*>(s.psi), pp, tiles, partials);
+            else
+                hipLaunchKernelGGL(k_expect_tile<float>, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+                                   reinterpret_cast<const float2*>(s.psi), pp, tiles, partials);
+            QFVM_LAUNCHED();
+            hipLaunchKernelGGL(k_term_sums, dim3((unsigned)pp.nterms), dim3(RED_BLOCK), 0, s.stream, partials, grid, pp,
+                               out_dev);
+            QFVM_LAUNCHED();
+        }
+    }
 }
 
 void reduce_probabilities(State& s, double* host_out) {

@@ -15,7 +15,7 @@ from ..elements.quantum_gate import CircuitWrapper, QuantumGate
 from ..exceptions import QuafuError
 from ..results.results import SimuResult
 from .program import KRAUS, UCHAN, compile_gates
-from .qfvm import _call, applyop_statevec, bitstring_counts, expect_statevec, simulate_circuit
+from .qfvm import _call, applyop_statevec, bitstring_counts, simulate_circuit
 
 
 MAX_DEVICE_MARGINAL = 12  # measured qubits reduced on the device while the final state is resident
@@ -83,8 +83,11 @@ class SVSimulator(Simulator):
         if marginals is not None:
             res_info["probabilities"] = marginals
         if hamiltonian:
+            # expect_statevec on the final state while it is still on the device (the engine pool
+            # keeps it), all terms in few passes, instead of uploading the returned copy again
             paulis = hamiltonian.paulis
-            res = expect_statevec(res_info["statevector"], paulis)
+            dev = _call(_native.pool.get, qc.num, self.precision)
+            res = [float(v) for v in _call(dev.expect, [(p.paulistr, list(p.pos)) for p in paulis])]
             for i, j in enumerate(paulis):
                 res[i] *= j.coeff
             res_info["pauli_expects"] = res

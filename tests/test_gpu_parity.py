@@ -74,6 +74,53 @@ def test_expectations(dtype):
     assert np.abs(dev.probabilities() - cpu.probabilities()).max() < (1e-14 if dtype == "c128" else 1e-6)
 
 
+def tfim_terms(n):
+    return [("ZZ", [i, (i + 1) % n]) for i in range(n)] + [("X", [i]) for i in range(n)]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [1, 3, 12, 20])
+def test_many_term_expectations(n, dtype):
+    """expect_statevec over many terms at once (LDS-tiled passes that each evaluate every term
+    whose X / Y qubits fit the tile) against the oracle's per-term expect_pauli: the periodic TFIM,
+    random k-local strings and strings too wide for one tile. A term that names a qubit twice is
+    refused (the reference adds its bit masks, which then carry past the last qubit)."""
+    rng = np.random.Generator(np.random.PCG64(700 + n))
+    psi = rand_state(n, rng)
+    terms = tfim_terms(n) if n > 1 else [("Z", [0]), ("X", [0]), ("Y", [0])]
+    for _ in range(40):
+        k = int(rng.integers(1, n + 1))
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        terms.append(("".join(rng.choice(list("IXYZ"), k)), qs))
+    if n >= 12:
+        terms.append(("X" * n, list(range(n))))
+        terms.append(("Y" * (n // 2) + "X" * (n - n // 2), list(range(n))[::-1]))
+    cpu = CpuState(n)
+    cpu.load(psi)
+    want = cpu.expect(terms)
+    dev = gpu_state(n, dtype)
+    dev.load(psi)
+    dev.profile(reset=True)
+    got = dev.expect(terms)
+    prof = dev.profile(reset=True)
+    err = float(np.abs(got - want).max())
+    print(f"[expect n={n} {dtype}] {len(terms)} terms, max-abs vs oracle {err:.2e}, launches "
+          f"{ {k: v['launches'] for k, v in prof.items()} }", flush=True)
+    assert err < (1e-12 if dtype == "c128" else 1e-5)
+    if n == 20:
+        # the TFIM alone: the Z strings and the X terms on the tile's 5 low qubits and 6 (complex128)
+        # or 7 (complex64) chosen ones share the first pass; the remaining X terms take two more
+        dev.profile(reset=True)
+        got = dev.expect(tfim_terms(n))
+        prof = dev.profile(reset=True)
+        assert np.abs(got - cpu.expect(tfim_terms(n))).max() < (1e-12 if dtype == "c128" else 1e-5)
+        assert prof["expect_tile"]["launches"] == 3 and "pauli_expect" not in prof
+    if n >= 3:
+        with pytest.raises(_native.EngineError) as e:
+            dev.expect([("XZX", [2, 0, 2])])
+        assert e.value.code == _native.ERR_VALUE and "repeated qubit" in str(e.value)
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n", [0, 1, 4, 5, 6, 12, 17])
 def test_sampler_bit_exact(n, dtype):
