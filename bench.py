@@ -12,7 +12,10 @@ Printed (one JSON line):
   gate_equiv     the unfused per-gate byte model (sum over gates of 2 * 16 B * 2^(n - controls)) /
                  step time: comparable with the CPU, may exceed the HBM peak once gates are fused
   cpu_baseline   the CPU oracle (a restatement of qfvm's OpenMP engine, built with its flags) on
-                 the first layer of the same circuit, same unit as gate_equiv
+                 the first 4 layers of the same circuit, every host core, median of 3 runs, same unit
+                 as gate_equiv; nproc and the CPU model are recorded
+  parity_vs_oracle  max-abs difference over all 2^n amplitudes between the GPU and the oracle after
+                 those 4 layers
 
 roofline.traffic is the HBM bytes per launch of the dominant kernel read from rocprofv3 FETCH_SIZE
 and WRITE_SIZE runs, each scaled by the same counter on a plain streaming kernel of known size
@@ -25,6 +28,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -55,29 +59,62 @@ def gate_equiv_bytes(prog, n: int, amp_bytes: int) -> float:
     return tot
 
 
-def cpu_baseline(prog, n: int, layer_gates: int, amp_bytes: int):
-    """Oracle (CPU restatement of qfvm) on the first `layer_gates` gates of the program."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    from oracle.cpu import CpuState
-    from pyquafu_amd.simulators.program import Program
+def host_cpu():
+    """(logical cores, CPU model) of this host, as nproc and lscpu report them."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 1, model
 
-    sub = Program(prog.ops[:layer_gates], prog.idx, prog.mat, n_qubits=n)
+
+def cpu_baseline(n: int, layers: int, crz: int, gpu_state=None, runs: int = 3):
+    """The oracle (the reference's CPU algorithm with its flags: one OpenMP sweep per gate) on the
+    first `layers` layers of the benchmark circuit, every host core, median of `runs` runs from |0>;
+    with `gpu_state`, the max-abs difference of every amplitude against it."""
+    cores, model = host_cpu()
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from oracle.cpu import CpuState
+    from pyquafu_amd import workloads
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    sub = compile_circuit(workloads.random_u3_cx(n, layers, crz, seed=20250002, measure=False))
     s = CpuState(n)
-    t0 = time.perf_counter()
-    s.apply(sub)
-    dt = time.perf_counter() - t0
+    times = []
+    for r in range(runs):
+        s.init_basis(0)
+        t0 = time.perf_counter()
+        s.apply(sub)
+        times.append(time.perf_counter() - t0)
+        print(f"[bench] CPU baseline run {r + 1}/{runs}: {times[-1]:.2f} s", file=sys.stderr, flush=True)
+    dt = float(np.median(times))
+    parity = None
+    if gpu_state is not None:
+        ref = s.view()
+        err = 0.0
+        for k in range(0, ref.size, 1 << 24):
+            err = max(err, float(np.abs(gpu_state[k:k + (1 << 24)] - ref[k:k + (1 << 24)]).max()))
+        parity = {"layers": layers, "gates": int(sub.n_ops), "amplitudes_compared": int(ref.size), "max_abs": err}
     by = gate_equiv_bytes(sub, n, 16)
     del s
+    threads = int(os.environ["OMP_NUM_THREADS"])
     return {
         "value": round(by / dt / 1e9, 3),
         "unit": "GB/s gate-equivalent",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {layer_gates} gates (one layer) of the same {n}-qubit circuit, complex128, "
-                  f"{dt:.2f} s, one OpenMP sweep per gate",
+        "sample": f"first {layers} layers ({sub.n_ops} gates) of the same {n}-qubit circuit, complex128, from "
+                  f"|0>, median of {runs} runs ({', '.join(f'{t:.2f}' for t in times)} s), one OpenMP sweep per gate",
         "seconds": round(dt, 3),
-    }
+        "host": {"nproc": cores, "model": model, "omp_threads": threads,
+                 "omp_proc_bind": os.environ.get("OMP_PROC_BIND")},
+    }, parity
 
 
 def traffic_from_profile(kernel: str, n: int, dtype: str):
@@ -361,6 +398,8 @@ def main(argv=None) -> int:
     ap.add_argument("--crz", type=int, default=15)
     ap.add_argument("--dtype", default="c128", choices=["c128", "c64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-layers", type=int, default=4,
+                    help="layers of the circuit the CPU baseline times (and the GPU state is compared on)")
     ap.add_argument("--no-fusion", action="store_true", help="one kernel per gate (the unfused path)")
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=0)
@@ -375,8 +414,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-circuit leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=4)
-    ap.add_argument("--graph-batch", action="store_true",
-                    help="also run the batch with every engine call as a captured hipGraph (not under a profiler)")
+    ap.add_argument("--no-graph-batch", action="store_true",
+                    help="skip the batch run with every engine call as a captured hipGraph (profiled runs skip it)")
     ap.add_argument("--qft-n", type=int, default=33)
     ap.add_argument("--qft-shots", type=int, default=1000000)
     args = ap.parse_args(argv)
@@ -495,7 +534,7 @@ def main(argv=None) -> int:
         out["adjoint"] = None
     if not args.no_batch:
         print(f"[bench] batch leg: {args.batch_count} circuits", file=sys.stderr, flush=True)
-        out["batch"] = batch_leg(args.batch_count, args.batch_workers, args.graph_batch)
+        out["batch"] = batch_leg(args.batch_count, args.batch_workers, not args.no_graph_batch)
     else:
         out["batch"] = None
     if not args.no_qft:
@@ -509,11 +548,19 @@ def main(argv=None) -> int:
     else:
         out["dense"] = None
     if not args.no_cpu:
-        per_layer = n + n // 2 + (n - 1) // 2 + args.crz
-        print(f"[bench] CPU baseline: {per_layer} gates on the oracle", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(prog, n, per_layer, amp)
+        # the same first layers on the GPU, every amplitude read back for the comparison
+        sub = compile_circuit(workloads.random_u3_cx(n, args.cpu_layers, args.crz, seed=20250002, measure=False))
+        gdev = _native.DeviceState(n, args.dtype)
+        gdev.apply(sub)
+        gstate = gdev.store()
+        gdev.close()
+        print(f"[bench] CPU baseline: {args.cpu_layers} layers ({sub.n_ops} gates) on the oracle", file=sys.stderr,
+              flush=True)
+        out["cpu_baseline"], out["parity_vs_oracle"] = cpu_baseline(n, args.cpu_layers, args.crz, gstate)
+        del gstate
     else:
         out["cpu_baseline"] = None
+        out["parity_vs_oracle"] = None
     print(json.dumps(out), flush=True)
     return 0
 

@@ -501,7 +501,11 @@ unsigned char* State::stage_dev(int k, size_t bytes) {
 void State::stage_upload(int k, size_t bytes) {
     Stage& st = stages[k];
     QFVM_HIP(hipMemcpyAsync(st.dev, st.cur_host, bytes, hipMemcpyHostToDevice, stream));
-    if (capturing) return;  // the graph's copy node reads the arena when the graph runs
+    if (capturing) {  // the graph's copy node reads the arena when the graph runs
+        const uint64_t rec[2] = {(uint64_t)k, (uint64_t)bytes};
+        for (size_t b = 0; b < sizeof(rec); b++) cap_sig = (cap_sig ^ reinterpret_cast<const unsigned char*>(rec)[b]) * 1099511628211ull;
+        return;
+    }
     if (!st.done) QFVM_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
     QFVM_HIP(hipEventRecord(st.done, stream));
 }
@@ -609,6 +613,36 @@ static uint64_t adjoint_key(const State& s, const qfvm_program* prog, const int6
     return h;
 }
 
+// Shape of a captured graph: node count and types, every kernel node's function, grid, block and
+// dynamic LDS bytes, the edge count, and (cap_sig, accumulated by stage_upload during the capture)
+// the size of every upload. The plan of a call depends on parameter values as well as on the
+// program's structure (an RX(0) plans as a diagonal gate, diagonal runs merge terms by value), so
+// two calls with one structure key may capture different graphs; a cached executable is updated
+// in place only from a capture of the same shape, which is what hipGraphExecUpdate supports.
+static uint64_t graph_signature(hipGraph_t g, uint64_t seed) {
+    size_t count = 0;
+    QFVM_HIP(hipGraphGetNodes(g, nullptr, &count));
+    std::vector<hipGraphNode_t> nodes(count);
+    if (count) QFVM_HIP(hipGraphGetNodes(g, nodes.data(), &count));
+    uint64_t h = fnv(seed, &count, sizeof(count));
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType t{};
+        QFVM_HIP(hipGraphNodeGetType(nd, &t));
+        const int32_t ti = (int32_t)t;
+        h = fnv(h, &ti, sizeof(ti));
+        if (t == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            QFVM_HIP(hipGraphKernelNodeGetParams(nd, &kp));
+            const uint64_t f[8] = {(uint64_t)(uintptr_t)kp.func, kp.gridDim.x, kp.gridDim.y, kp.gridDim.z,
+                                   kp.blockDim.x, kp.blockDim.y, kp.blockDim.z, kp.sharedMemBytes};
+            h = fnv(h, f, sizeof(f));
+        }
+    }
+    size_t edges = 0;
+    QFVM_HIP(hipGraphGetEdges(g, nullptr, nullptr, &edges));
+    return fnv(h, &edges, sizeof(edges));
+}
+
 // dense multi-target gates stage their matrix from pageable host memory: such calls run eagerly
 static bool graph_safe(const std::vector<Gate>& gates) {
     for (const Gate& g : gates)
@@ -639,6 +673,7 @@ static bool run_graph(State& s, uint64_t key, F&& enqueue) {
         s.cap_host_bytes = s.cap_need;
     }
     s.cap_off = 0;
+    s.cap_sig = 1469598103934665603ull;
     hipGraph_t graph = nullptr;
     QFVM_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     s.capturing = true;
@@ -664,10 +699,15 @@ static bool run_graph(State& s, uint64_t key, F&& enqueue) {
             if (g) hipGraphDestroy(g);
         }
     } guard{graph};
+    const uint64_t sig = graph_signature(graph, s.cap_sig);
     State::GraphEntry* entry = nullptr;
     for (State::GraphEntry& e : s.graph_cache)
         if (e.key == key) entry = &e;
     bool ready = false;
+    if (entry && entry->exec && entry->sig != sig) {  // same structure key, different plan: start afresh
+        hipGraphExecDestroy(entry->exec);
+        entry->exec = nullptr;
+    }
     if (entry && entry->exec) {
         hipGraphNode_t bad = nullptr;
         hipGraphExecUpdateResult res = hipGraphExecUpdateError;
@@ -687,10 +727,11 @@ static bool run_graph(State& s, uint64_t key, F&& enqueue) {
             if (s.graph_cache[lru].exec) hipGraphExecDestroy(s.graph_cache[lru].exec);
             s.graph_cache.erase(s.graph_cache.begin() + (std::ptrdiff_t)lru);
         }
-        s.graph_cache.push_back(State::GraphEntry{key, 0, nullptr});
+        s.graph_cache.push_back(State::GraphEntry{key, sig, 0, nullptr});
         entry = &s.graph_cache.back();
     }
     if (!ready) {
+        entry->sig = sig;
         QFVM_HIP(hipGraphInstantiate(&entry->exec, graph, nullptr, nullptr, 0));
         s.kstats[(size_t)s.stat_index("graph_instantiate")].second.launches++;
     }

@@ -162,8 +162,10 @@ struct State {
     size_t cap_host_bytes = 0;
     size_t cap_off = 0;
     size_t cap_need = 0;  // arena size an abandoned capture asked for
+    uint64_t cap_sig = 0;  // uploads of the running capture (slot and size of each), hashed
     struct GraphEntry {
         uint64_t key;
+        uint64_t sig;  // shape of the graph the executable was made from
         uint64_t last_use;
         hipGraphExec_t exec;
     };

@@ -192,3 +192,43 @@ def test_graph_replays_with_checkpoints():
     finally:
         eager.close()
         graph.close()
+
+
+def test_value_dependent_plan_gets_its_own_executable():
+    """RX(0) plans as a diagonal gate, RX(0.3) as a dense one: one structure key, two different
+    captured graphs. The second call must not update the first call's executable in place (that
+    needs the same kernels, grids and LDS sizes); it instantiates a graph of its own, and every
+    call matches the eager engine and the oracle."""
+    from pyquafu_amd.elements import RXGate
+
+    n = 11
+
+    def circuit(theta):
+        qc = QuantumCircuit(n)
+        for q in range(n):
+            qc << RYGate(q, Parameter(f"y{q}", 0.1 * (q + 1)))
+        for q in range(n):
+            qc << RXGate(q, Parameter(f"x{q}", theta))
+        for q in range(n):
+            qc << CZGate(q, (q + 1) % n)
+        return qc
+
+    eager = _native.DeviceState(n, "c128")
+    graph = _native.DeviceState(n, "c128")
+    try:
+        graph.set_option("graphs", 1)
+        ham = tfim(n)
+        for theta in (0.0, 0.3, 0.0, 0.3):
+            args = call_args(circuit(theta), ham)
+            g0, e0 = run(eager, args)
+            g1, e1 = run(graph, args)
+            assert np.array_equal(g1, g0) and e1 == e0, theta
+            want, e_want = og.adjoint(args[0], n, *args[1:])
+            assert np.abs(g1 - want).max() < 1e-10
+        c = graph_counts(graph)
+        # (the first call on a fresh handle grows its buffers and so runs eagerly)
+        assert c["graph_launch"] >= 3
+        assert c["graph_instantiate"] >= 2  # the 0.3 capture never updates the 0.0 executable
+    finally:
+        eager.close()
+        graph.close()

@@ -112,3 +112,21 @@ def test_qft_and_ansatz_plans():
     check(prog, n)
     qc, _ = workloads.hea_ansatz(n, 3, parametrised=False)
     check(compile_circuit(qc), n)
+
+
+def test_plan_depends_on_parameter_values():
+    """The fused plan of RX(0) differs from that of RX(0.3) (a zero-angle rotation is diagonal and
+    takes no tile slot), so a cache keyed on program structure alone must also compare plans: graph
+    mode compares the captured graphs' shapes before it updates an executable."""
+    from pyquafu_amd import _native
+    from pyquafu_amd.simulators.program import RX, ProgramBuilder
+
+    def prog(theta):
+        b = ProgramBuilder()
+        for q in range(14):
+            b.add(RX, [q], param=theta)
+        return b.program(n_qubits=14)
+
+    p0 = _native.plan_json(prog(0.0), 14, "c128")
+    p1 = _native.plan_json(prog(0.3), 14, "c128")
+    assert p0 != p1
