@@ -165,6 +165,18 @@ int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, con
                        const int32_t* len, int64_t n_terms, double* out);
 
 /*
+ * qfvm_expect_paulis without any allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device or pinned memory (code under HIP graph capture setup, tight serving loops,
+ * harnesses that forbid hipMalloc). Same terms, same results (the same kernels, the same summation
+ * order) as qfvm_expect_paulis, but every scratch byte lives in a fixed 530 KiB device array that
+ * the library holds per GPU until the process ends: at most 256 terms per call (more ->
+ * QFVM_ERR_VALUE). Calls on one GPU share that array, so calls that overlap on one GPU (two threads,
+ * two handles) race; serialise them. The state must not be modified concurrently either.
+ */
+int qfvm_expect_paulis_noalloc(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
+                               const int32_t* len, int64_t n_terms, double* out);
+
+/*
  * <psi|H|psi> for H = sum_t coef_t P_t (coef as re, im pairs; terms laid out as in
  * qfvm_expect_paulis, no qubit repeated within a term): the real part of the sum that
  * SVSimulator.run forms from expect_statevec and the coefficients (simulator.py:133-139), taken in

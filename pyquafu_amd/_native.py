@@ -61,6 +61,7 @@ SIGNATURES = {
     "qfvm_apply": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram)]),
     "qfvm_run": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram), _vp, _i64, _vp, _i32, _i32]),
     "qfvm_expect_paulis": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "qfvm_expect_paulis_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "qfvm_expect_hamiltonian": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "qfvm_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(CProgram), _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64,
                                     _vp, _vp]),

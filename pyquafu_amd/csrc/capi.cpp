@@ -900,6 +900,21 @@ int qfvm_expect_paulis(qfvm_handle h, const char* chars, const int32_t* pos, con
     });
 }
 
+int qfvm_expect_paulis_noalloc(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
+                               const int32_t* len, int64_t n_terms, double* out) {
+    return guarded([&] {
+        State* s = need(h);
+        if (n_terms < 0) throw Error(QFVM_ERR_VALUE, "negative term count");
+        if (n_terms > EXPECT_BATCH)
+            throw Error(QFVM_ERR_VALUE, "qfvm_expect_paulis_noalloc takes at most 256 terms (its scratch is fixed)");
+        if (n_terms == 0) return;
+        if (!chars || !pos || !off || !len || !out) throw Error(QFVM_ERR_VALUE, "null term array");
+        PauliHost terms[EXPECT_BATCH];
+        for (int64_t t = 0; t < n_terms; t++) terms[t] = parse_pauli(*s, chars, pos, off[t], len[t], (int)t);
+        expect_paulis_noalloc(*s, terms, (int)n_terms, out);
+    });
+}
+
 int qfvm_expect_hamiltonian(qfvm_handle h, const char* chars, const int32_t* pos, const int64_t* off,
                             const int32_t* len, const double* coef, int64_t n_terms, double* out) {
     return guarded([&] {

@@ -257,6 +257,9 @@ constexpr size_t EXPECT_PARTIALS = 32 * 2048;
 constexpr size_t RED_PARTIALS = 2048 + 1;
 void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* out_dev, double* partials,
                         double* fallback, bool account = true);
+// queue_expectations on the library's fixed device scratch, results copied to host_out (n_terms
+// doubles); allocates nothing. n_terms <= EXPECT_BATCH.
+void expect_paulis_noalloc(State& s, const PauliHost* terms, int n_terms, double* host_out);
 void reduce_marginal(State& s, const std::vector<int>& qbits, std::vector<double>& probs);
 void launch_collapse(State& s, const std::vector<int>& qbits, uint64_t meas_state, double scale);
 void launch_swap_states(State& s, const std::vector<int>& qbits, uint64_t a, uint64_t b);

@@ -392,6 +392,39 @@ void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* o
     }
 }
 
+// Allocation-free expectations (qfvm_expect_paulis_noalloc). Its scratch is this file-scope device
+// array: per-workgroup partials, the wide-term fallback's partials and the results, 530 KiB. Every
+// GPU the library runs on holds its own copy until the process ends, and every call on that GPU
+// shares it (calls that overlap on one GPU race). Host code never names the array in a kernel
+// argument or copy: it uses the address hipGetSymbolAddress returns, resolved once per device with
+// that device current.
+constexpr size_t NOALLOC_DOUBLES = EXPECT_PARTIALS + RED_PARTIALS + EXPECT_BATCH;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+static double* noalloc_scratch(int device) {
+    static std::mutex lock;
+    static double* ptr[64] = {};
+    std::lock_guard<std::mutex> guard(lock);
+    if (device < 0 || device >= 64) throw Error(QFVM_ERR_DEVICE, "device index out of range");
+    if (!ptr[device]) {
+        void* p = nullptr;
+        QFVM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[device] = static_cast<double*>(p);
+    }
+    return ptr[device];
+}
+
+void expect_paulis_noalloc(State& s, const PauliHost* terms, int n_terms, double* host_out) {
+    if (n_terms > EXPECT_BATCH) throw Error(QFVM_ERR_VALUE, "at most 256 terms per allocation-free call");
+    double* base = noalloc_scratch(s.device);
+    double* partials = base;
+    double* fallback = base + EXPECT_PARTIALS;
+    double* out_dev = fallback + RED_PARTIALS;
+    queue_expectations(s, terms, n_terms, out_dev, partials, fallback, false);
+    QFVM_HIP(hipMemcpyAsync(host_out, out_dev, (size_t)n_terms * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    QFVM_HIP(hipStreamSynchronize(s.stream));
+}
+
 void reduce_probabilities(State& s, double* host_out) {
     const uint64_t chunk = 1ull << 23;
     double* stage = reinterpret_cast<double*>(s.ensure_scratch(std::min(s.size, chunk) * sizeof(double)));
