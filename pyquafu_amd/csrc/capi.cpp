@@ -588,7 +588,7 @@ static uint64_t adjoint_key(const State& s, const qfvm_program* prog, const int6
                             const int32_t* len, int64_t n_terms) {
     uint64_t h = 1469598103934665603ull;
     const int32_t opts[] = {s.n, s.dtype, s.mfma, s.fusion, s.tile_bits, s.low_bits, s.max_groups, s.pass_cost, s.split_rows,
-                            s.stream_tiles, s.stagger, s.stagger_shift};
+                            s.stream_tiles, s.stagger, s.stagger_shift, s.rows8};
     h = fnv(h, opts, sizeof(opts));
     h = fnv(h, &prog->n_ops, sizeof(prog->n_ops));
     for (int64_t k = 0; k < prog->n_ops; k++) {
@@ -1134,6 +1134,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->pass_cost = (int)value;
         } else if (k == "split_rows") {
             s->split_rows = value != 0;
+        } else if (k == "rows8") {
+            s->rows8 = value != 0;
         } else if (k == "stream") {
             s->stream_tiles = value != 0;
         } else if (k == "stagger") {
@@ -1174,6 +1176,9 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
         cfg.low_bits = low_bits > 0 ? low_bits : default_low_bits(tmp.dtype, 1);
         cfg.max_groups = max_groups > 0 ? max_groups : 12;
         cfg.max_cost = pass_cost > 0 ? pass_cost : 0;
+        // the group width qfvm_apply would use with default options (complex128: rows of 8)
+        const int bb = std::min(cfg.tile_bits, n);
+        if (tmp.dtype == QFVM_C128 && bb >= 10 && bb <= 13) cfg.group_bits = 3;
         std::vector<PlanItem> plan = plan_gates(gates, n, cfg);
         std::string j = "{\"items\": [";
         char t[512];
@@ -1185,7 +1190,7 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
             const PlanItem& it = plan[i];
             j += i ? ", " : "";
             if (it.fused) {
-                j += "{\"fused\": true, \"Q\": [";
+                j += "{\"fused\": true, \"group_bits\": " + std::to_string(it.pass.group_bits) + ", \"Q\": [";
                 for (size_t k = 0; k < it.pass.Q.size(); k++) j += (k ? ", " : "") + std::to_string(it.pass.Q[k]);
                 j += "], \"groups\": [";
                 for (size_t g = 0; g < it.pass.groups.size(); g++) {

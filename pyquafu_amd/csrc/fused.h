@@ -69,7 +69,7 @@ struct FOpHost {
 // A group is either a compute group (4 tile qubits and their gates) or, in the two-state reverse
 // sweep, a stage: gradient markers evaluated on the tiles of both states between two compute groups.
 struct FGroup {
-    int S[FUSE_GROUP_BITS];  // global qubits of the group (ascending); unused by a stage
+    int S[FUSE_GROUP_BITS] = {-1, -1, -1, -1};  // global qubits of the group (ascending); unused by a stage
     bool stage = false;
     std::vector<FOpHost> ops;
     uint64_t pxm[FUSE_GROUP_BITS] = {0, 0, 0, 0};  // relabel masks at the end of the group
@@ -79,6 +79,7 @@ struct FGroup {
 struct FPass {
     std::vector<int> Q;          // tile qubits, ascending; Q[0..L-1] = 0..L-1
     std::vector<FGroup> groups;
+    int group_bits = FUSE_GROUP_BITS;  // qubits per compute group (3: rows of 8, 4: rows of 16)
 };
 
 struct FuseConfig {
@@ -86,6 +87,7 @@ struct FuseConfig {
     int low_bits;    // L
     int max_groups;
     int max_cost;    // compute budget of a pass in quarter dense-gate units (0: unlimited)
+    int group_bits = FUSE_GROUP_BITS;  // qubits per compute group: 3 or 4
 };
 
 // Default tile shape per precision and state count (states = 2: the adjoint's two-state sweep, both

@@ -992,6 +992,340 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
 }
 
 // ------------------------------------------------------------------------------------------------
+// Row-of-8 pass kernel (complex128 one-state passes of 2^10 .. 2^13-amplitude tiles, 3-qubit
+// groups). One lane holds the 8
+// amplitudes that differ only in its group's 3 bits, so no gate needs another lane's data (the
+// split-row kernel pairs lanes for its 4th group bit, and each of those gates pays 32 DPP moves);
+// a 2^b tile still gets 2^(b-3) threads, and the load / store skeleton is the split-row kernel's.
+// The op loop alternates two register sets (a -> o, then o -> a), so no op copies its results
+// back, and the common ops -- a 2x2 on one group bit whose coefficients are the same for every
+// lane -- keep their coefficients in scalar registers: measured on the 30-qubit shape, 12 to 18
+// such gates per pass cost almost nothing over the bare load / store (the arithmetic overlaps
+// the HBM traffic). Device op kinds (FOpDev.kind):
+enum K8Kind : int32_t {
+    K8_U2 = 0,  // complex 2x2 on group bit g, the same for every lane; pairs outside the group
+                //   controls cm_group keep their values
+    K8_R2 = 1,  // the same with a real matrix
+    K8_G2 = 2,  // complex 2x2 whose coefficients differ per lane: a pending relabel swaps them
+                //   (FL_TARGET_PX), controls outside the group (cm_ext) make it the identity
+    K8_DG = 3,  // diagonal: diag(d0, d1) on group bit g, or (g = -1 - q) on qubit q outside the
+                //   group, with group / outside controls and relabel swaps
+    K8_FL = 4,  // relabel flush of group bit g (pairs swap where the lane's parity is set)
+    K8_DR = 5   // diagonal run (the split-row kernel's records, subsets of 3 bits)
+};
+
+// An op record as the row-of-8 kernel holds it: the header (kind, g, cm_group, flags), the
+// outside-control and relabel masks and the 4 coefficients, fetched with wide scalar loads so
+// that the next op's record can be in flight while the current op computes.
+template <typename R>
+struct Op8 {
+    int32_t kind, g, cm, flags;
+    uint64_t cm_ext, pxmask;
+    R m[8];
+};
+template <typename R>
+__device__ __forceinline__ Op8<R> load_op8(const FOpDev<R>* __restrict__ p) {
+    static_assert(sizeof(FOpDev<R>) == 32 + 8 * sizeof(R), "op record layout");
+    Op8<R> o;
+    const int4 h = *reinterpret_cast<const int4*>(p);
+    o.kind = h.x;
+    o.g = h.y;
+    o.cm = h.z;
+    o.flags = h.w;
+    const ulonglong2 mk2 = *reinterpret_cast<const ulonglong2*>(reinterpret_cast<const char*>(p) + 16);
+    o.cm_ext = mk2.x;
+    o.pxmask = mk2.y;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o.m[k] = p->m[k];
+    return o;
+}
+
+// 2x2 on group bit G for the pairs whose group-control bits CM are set (CM a compile-time subset
+// of the other two bits); the other pairs are copied
+template <typename R, int G, int CM, bool REAL, int M>
+__device__ __forceinline__ void r8_pair(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11) {
+    if constexpr (!((M >> G) & 1)) {
+        constexpr int M1 = M | (1 << G);
+        if constexpr ((M & CM) != CM) {
+            o[M] = a[M];
+            o[M1] = a[M1];
+        } else {
+            const V2<R> x = a[M], y = a[M1];
+            if constexpr (REAL) {
+                o[M] = mk<R>(m00.x * x.x + m01.x * y.x, m00.x * x.y + m01.x * y.y);
+                o[M1] = mk<R>(m10.x * x.x + m11.x * y.x, m10.x * x.y + m11.x * y.y);
+            } else {
+                o[M] = cmul2<R>(m00, x, m01, y);
+                o[M1] = cmul2<R>(m10, x, m11, y);
+            }
+        }
+    }
+}
+
+template <typename R, int G, int CM, bool REAL, int... Ms>
+__device__ __forceinline__ void r8_2x2(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11,
+                                       std::integer_sequence<int, Ms...>) {
+    (r8_pair<R, G, CM, REAL, Ms>(a, o, m00, m01, m10, m11), ...);
+}
+
+// the group-control masks of a gate on bit G: subsets of the other two bits
+template <typename R, int G, bool REAL>
+__device__ __forceinline__ void r8_ctrl(const V2<R> (&a)[8], V2<R> (&o)[8], int cm, V2<R> m00, V2<R> m01, V2<R> m10,
+                                        V2<R> m11) {
+    constexpr int B0 = G == 0 ? 2 : 1, B1 = G == 2 ? 2 : 4;
+    constexpr auto seq = std::make_integer_sequence<int, 8>{};
+    if (cm == 0)
+        r8_2x2<R, G, 0, REAL>(a, o, m00, m01, m10, m11, seq);
+    else if (cm == B0)
+        r8_2x2<R, G, B0, REAL>(a, o, m00, m01, m10, m11, seq);
+    else if (cm == B1)
+        r8_2x2<R, G, B1, REAL>(a, o, m00, m01, m10, m11, seq);
+    else
+        r8_2x2<R, G, B0 | B1, REAL>(a, o, m00, m01, m10, m11, seq);
+}
+
+template <typename R, bool REAL>
+__device__ __forceinline__ void r8_bit(const V2<R> (&a)[8], V2<R> (&o)[8], int g, int cm, V2<R> m00, V2<R> m01, V2<R> m10,
+                                       V2<R> m11) {
+    switch (g) {
+        case 0: r8_ctrl<R, 0, REAL>(a, o, cm, m00, m01, m10, m11); break;
+        case 1: r8_ctrl<R, 1, REAL>(a, o, cm, m00, m01, m10, m11); break;
+        default: r8_ctrl<R, 2, REAL>(a, o, cm, m00, m01, m10, m11); break;
+    }
+}
+
+// diagonal factors: member M gets f1 where its bit G is set (G = -1: f0 everywhere), where its
+// group-control bits CM are set; the others are copied
+template <typename R, int G, int CM, int M>
+__device__ __forceinline__ void r8_scale(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R> f0, V2<R> f1) {
+    if constexpr ((M & CM) != CM) {
+        o[M] = a[M];
+    } else if constexpr (G >= 0 && ((M >> G) & 1)) {
+        o[M] = cmul<R>(f1, a[M]);
+    } else {
+        o[M] = cmul<R>(f0, a[M]);
+    }
+}
+
+template <typename R, int G, int CM, int... Ms>
+__device__ __forceinline__ void r8_diag_cm(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R> f0, V2<R> f1,
+                                           std::integer_sequence<int, Ms...>) {
+    (r8_scale<R, G, CM, Ms>(a, o, f0, f1), ...);
+}
+
+template <typename R, int G>
+__device__ __forceinline__ void r8_diag(const V2<R> (&a)[8], V2<R> (&o)[8], int cm, V2<R> f0, V2<R> f1) {
+    constexpr auto seq = std::make_integer_sequence<int, 8>{};
+    switch (cm) {  // any subset of the 3 group bits (never the target's own bit)
+        case 0: r8_diag_cm<R, G, 0>(a, o, f0, f1, seq); break;
+        case 1: r8_diag_cm<R, G, 1>(a, o, f0, f1, seq); break;
+        case 2: r8_diag_cm<R, G, 2>(a, o, f0, f1, seq); break;
+        case 3: r8_diag_cm<R, G, 3>(a, o, f0, f1, seq); break;
+        case 4: r8_diag_cm<R, G, 4>(a, o, f0, f1, seq); break;
+        case 5: r8_diag_cm<R, G, 5>(a, o, f0, f1, seq); break;
+        case 6: r8_diag_cm<R, G, 6>(a, o, f0, f1, seq); break;
+        default: r8_diag_cm<R, G, 7>(a, o, f0, f1, seq); break;
+    }
+}
+
+template <typename R, int G, int M>
+__device__ __forceinline__ void r8_flush_pair(const V2<R> (&a)[8], V2<R> (&o)[8], bool sw) {
+    if constexpr (!((M >> G) & 1)) {
+        constexpr int M1 = M | (1 << G);
+        o[M] = csel<R>(sw, a[M1], a[M]);
+        o[M1] = csel<R>(sw, a[M], a[M1]);
+    }
+}
+
+template <typename R, int G, int... Ms>
+__device__ __forceinline__ void r8_flush(const V2<R> (&a)[8], V2<R> (&o)[8], bool sw, std::integer_sequence<int, Ms...>) {
+    (r8_flush_pair<R, G, Ms>(a, o, sw), ...);
+}
+
+// factor f_S of a diagonal run applied to the members M that contain S (terms as in run_subset)
+template <typename R, int S, int... Ms>
+__device__ __forceinline__ void r8_run_subset(V2<R> (&v)[8], const RunHead<R>* __restrict__ rh,
+                                              const RunTerm<R>* __restrict__ terms, uint32_t active, uint32_t angles,
+                                              uint64_t G0, std::integer_sequence<int, Ms...>) {
+    if (!((active >> S) & 1u)) return;
+    V2<R> f = mk<R>(rh->f0[2 * S], rh->f0[2 * S + 1]);
+    R th = 0;
+    const int t1 = rh->tbeg[S + 1];
+    for (int t = rh->tbeg[S]; t < t1; t++) {
+        const RunTerm<R>& tm = terms[t];
+        if (tm.angle == 2) {
+            th += tm.v[0] * (R)(G0 & tm.cl);
+            continue;
+        }
+        const bool cond = (G0 & tm.cl) == tm.cl;
+        const bool sel = tm.pm ? (bool)px_bit(G0, tm.pm, tm.pc) : (bool)(tm.pc & 1u);
+        if (tm.angle) {
+            th += cond ? (sel ? tm.v[1] : tm.v[0]) : (R)0;
+        } else {
+            V2<R> val = csel<R>(sel, mk<R>(tm.v[2], tm.v[3]), mk<R>(tm.v[0], tm.v[1]));
+            val = csel<R>(cond, val, mk<R>(1, 0));
+            f = cmul<R>(f, val);
+        }
+    }
+    if ((angles >> S) & 1u) {
+        R sn, cs;
+        sin_cos(th, &sn, &cs);
+        f = cmul<R>(f, mk<R>(cs, sn));
+    }
+    ((((Ms & S) == S) ? (void)(v[Ms] = cmul<R>(f, v[Ms])) : (void)0), ...);
+}
+
+template <typename R, int... Ss>
+__device__ __forceinline__ void r8_run(const V2<R> (&a)[8], V2<R> (&o)[8], const unsigned char* __restrict__ runs,
+                                       const Op8<R>& op, uint64_t G0, std::integer_sequence<int, Ss...>) {
+    const RunHead<R>* rh = reinterpret_cast<const RunHead<R>*>(runs + op.cm_ext);
+    const RunTerm<R>* terms = reinterpret_cast<const RunTerm<R>*>(rh + 1);
+#pragma unroll
+    for (int m = 0; m < 8; m++) o[m] = a[m];
+    (r8_run_subset<R, Ss>(o, rh, terms, (uint32_t)op.g, (uint32_t)op.flags, G0, std::make_integer_sequence<int, 8>{}),
+     ...);
+}
+
+template <typename R>
+__device__ __forceinline__ void r8_op(const V2<R> (&a)[8], V2<R> (&o)[8], const Op8<R>& op, uint64_t G0,
+                                      const unsigned char* __restrict__ runs) {
+    switch (op.kind) {
+        case K8_U2:
+            r8_bit<R, false>(a, o, op.g, op.cm, mk<R>(op.m[0], op.m[1]), mk<R>(op.m[2], op.m[3]), mk<R>(op.m[4], op.m[5]),
+                             mk<R>(op.m[6], op.m[7]));
+            break;
+        case K8_R2:
+            r8_bit<R, true>(a, o, op.g, op.cm, mk<R>(op.m[0], 0), mk<R>(op.m[2], 0), mk<R>(op.m[4], 0),
+                            mk<R>(op.m[6], 0));
+            break;
+        case K8_G2: {
+            V2<R> m00 = mk<R>(op.m[0], op.m[1]), m01 = mk<R>(op.m[2], op.m[3]);
+            V2<R> m10 = mk<R>(op.m[4], op.m[5]), m11 = mk<R>(op.m[6], op.m[7]);
+            if (op.flags & FL_TARGET_PX) {  // registers hold the pair swapped where the relabel bit is set
+                const bool sw = px_bit(G0, op.pxmask, ((uint32_t)op.flags >> 1) & 1u);
+                const V2<R> t00 = m00, t01 = m01, t10 = m10, t11 = m11;
+                m00 = csel<R>(sw, t11, t00);
+                m01 = csel<R>(sw, t10, t01);
+                m10 = csel<R>(sw, t01, t10);
+                m11 = csel<R>(sw, t00, t11);
+            }
+            const bool ok = (G0 & op.cm_ext) == op.cm_ext;
+            m00 = csel<R>(ok, m00, mk<R>(1, 0));
+            m01 = csel<R>(ok, m01, mk<R>(0, 0));
+            m10 = csel<R>(ok, m10, mk<R>(0, 0));
+            m11 = csel<R>(ok, m11, mk<R>(1, 0));
+            r8_bit<R, false>(a, o, op.g, op.cm, m00, m01, m10, m11);
+            break;
+        }
+        case K8_DG: {
+            const V2<R> d0 = mk<R>(op.m[0], op.m[1]), d1 = mk<R>(op.m[6], op.m[7]);
+            // an outside target picks the factor by the lane's index bit; a pending relabel of an
+            // inside target swaps the two
+            const bool outside = op.g < 0;
+            const bool sel = outside ? (bool)((G0 >> (-1 - op.g)) & 1ull)
+                                     : ((op.flags & FL_TARGET_PX)
+                                            ? (bool)px_bit(G0, op.pxmask, ((uint32_t)op.flags >> 1) & 1u)
+                                            : false);
+            const bool ok = (G0 & op.cm_ext) == op.cm_ext;
+            V2<R> f0 = csel<R>(sel, d1, d0), f1 = csel<R>(sel, d0, d1);
+            f0 = csel<R>(ok, f0, mk<R>(1, 0));
+            f1 = csel<R>(ok, f1, mk<R>(1, 0));
+            switch (op.g) {
+                case 0: r8_diag<R, 0>(a, o, op.cm, f0, f1); break;
+                case 1: r8_diag<R, 1>(a, o, op.cm, f0, f1); break;
+                case 2: r8_diag<R, 2>(a, o, op.cm, f0, f1); break;
+                default: r8_diag<R, -1>(a, o, op.cm, f0, f1); break;
+            }
+            break;
+        }
+        case K8_FL: {
+            const bool sw = px_bit(G0, op.pxmask, ((uint32_t)op.flags >> 1) & 1u);
+            switch (op.g) {
+                case 0: r8_flush<R, 0>(a, o, sw, std::make_integer_sequence<int, 8>{}); break;
+                case 1: r8_flush<R, 1>(a, o, sw, std::make_integer_sequence<int, 8>{}); break;
+                default: r8_flush<R, 2>(a, o, sw, std::make_integer_sequence<int, 8>{}); break;
+            }
+            break;
+        }
+        default: r8_run<R>(a, o, runs, op, G0, std::make_integer_sequence<int, 8>{}); break;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024) k_pass8(V2<R>* __restrict__ psi, const PassDesc pd,
+                                                const FOpDev<R>* __restrict__ ops, const unsigned char* __restrict__ runs) {
+    extern __shared__ unsigned char lds_raw[];
+    V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
+    const int T = 1 << pd.b;
+    const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 8: one row of 8 per thread
+    const uint64_t base = tile_base(blockIdx.x, pd);
+    uint64_t* hi_tab = reinterpret_cast<uint64_t*>(tile + T);  // global offsets of the high local bits
+    const int L = pd.L;
+    {
+        // as k_fused_half: slot lh + tid receives element lh + swz(tid) (direct HBM -> LDS loads),
+        // the 8 loads cover the tile's 3 highest local bits, issued at a raised wave priority
+        __builtin_amdgcn_s_setprio(2);
+        const uint64_t g_sw = base | local_to_global_low(swz(tid), pd);
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const void* src = (const void*)(psi + (g_sw | pd.top[j]));
+            auto* dst = (__attribute__((address_space(3))) void*)(tile + j * NT + wave * 64);
+            __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
+        }
+        __builtin_amdgcn_s_setprio(0);
+    }
+    for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int gi = 0; gi < pd.ngroups; gi++) {
+        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2];
+        const int l0 = ins_bit(ins_bit(ins_bit(tid, s0), s1), s2);
+        const int sl = swz(l0);
+        const uint64_t G0 = base | (uint64_t)(l0 & ((1 << L) - 1)) | hi_tab[l0 >> L];
+        V2<R> A[8], B[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) A[m] = tile[sl ^ swz(((m & 1) << s0) | (((m >> 1) & 1) << s1) | (((m >> 2) & 1) << s2))];
+        int oi = pd.gop[gi];
+        const int o_end = pd.gop[gi + 1];
+        if (oi < o_end) {
+            // two ops per iteration (a -> b, b -> a); the next records load while these compute
+            Op8<R> cur = load_op8<R>(ops + oi);
+            for (;;) {
+                const Op8<R> nxt = load_op8<R>(ops + min(oi + 1, o_end - 1));
+                r8_op<R>(A, B, cur, G0, runs);
+                if (oi + 1 >= o_end) {
+#pragma unroll
+                    for (int m = 0; m < 8; m++) A[m] = B[m];
+                    break;
+                }
+                cur = load_op8<R>(ops + min(oi + 2, o_end - 1));
+                r8_op<R>(B, A, nxt, G0, runs);
+                oi += 2;
+                if (oi >= o_end) break;
+            }
+        }
+        // the group's relabeling, applied by the scatter (register m goes to logical m ^ px)
+        const uint32_t pc = pd.gpxc[gi];
+        int px = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const uint64_t msk = pd.gpxm[gi][j];
+            const int bit = msk ? px_bit(G0, msk, pc >> j) : (int)((pc >> j) & 1u);
+            px |= bit << pd.gbits[gi][j];
+        }
+        const int sx = sl ^ swz(px);
+#pragma unroll
+        for (int m = 0; m < 8; m++) tile[sx ^ swz(((m & 1) << s0) | (((m >> 1) & 1) << s1) | (((m >> 2) & 1) << s2))] = A[m];
+        __syncthreads();
+    }
+    const uint64_t g_tid = base | local_to_global_low(tid, pd);
+    __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+    for (int j = 0; j < 8; j++) store_nt<R>(&psi[g_tid | pd.top[j]], tile[swz(j * NT + tid)]);
+}
+
+// ------------------------------------------------------------------------------------------------
 // Appends the record of the diagonal run `run` of group G to `runs` (16-byte aligned) and returns
 // the device op that applies it.
 template <typename R>
@@ -1141,7 +1475,9 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
             if ((j >> i) & 1) pd.top16[j] |= 1ull << p.Q[b - 4 + i];
     }
     pd.ngroups = (int)p.groups.size();
-    if (pd.ngroups > FUSE_MAX_GROUPS || b > FUSE_MAX_TILE_BITS || b < FUSE_GROUP_BITS || b > s.n)
+    const int gbn = p.group_bits;
+    if (pd.ngroups > FUSE_MAX_GROUPS || b > FUSE_MAX_TILE_BITS || b < gbn || b > s.n || gbn < 3 ||
+        gbn > FUSE_GROUP_BITS)
         throw Error(QFVM_ERR_RUNTIME, "fused pass exceeds kernel limits");
     int local_of[64];
     for (int k = 0; k < 64; k++) local_of[k] = -1;
@@ -1170,12 +1506,13 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
             }
             continue;
         }
-        for (int j = 0; j < FUSE_GROUP_BITS; j++) {
-            if (local_of[G.S[j]] < 0) throw Error(QFVM_ERR_RUNTIME, "group qubit outside the tile");
+        for (int j = 0; j < gbn; j++) {
+            if (G.S[j] < 0 || local_of[G.S[j]] < 0) throw Error(QFVM_ERR_RUNTIME, "group qubit outside the tile");
             pd.gbits[gi][j] = local_of[G.S[j]];
         }
+        for (int j = gbn; j < FUSE_GROUP_BITS; j++) pd.gbits[gi][j] = 0;
         pd.gop[gi] = (int)h.size() - first;
-        for (int j = 0; j < FUSE_GROUP_BITS; j++) pd.gpxm[gi][j] = G.pxm[j];
+        for (int j = 0; j < FUSE_GROUP_BITS; j++) pd.gpxm[gi][j] = j < gbn ? G.pxm[j] : 0;
         pd.gpxc[gi] = G.pxc;
         int done_run = 0;
         for (const FOpHost& o : G.ops) {
@@ -1183,6 +1520,7 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
             if (o.run > 0) {                    // runs are numbered in op order
                 if (o.run > done_run) {
                     h.push_back(encode_run<R>(G, o.run, runs));
+                    if (gbn == 3) h.back().kind = K8_DR;
                     done_run = o.run;
                 }
                 continue;
@@ -1197,6 +1535,21 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
             for (int k = 0; k < 4; k++) {
                 d.m[2 * k] = (R)o.m[k].real();
                 d.m[2 * k + 1] = (R)o.m[k].imag();
+            }
+            if (gbn == 3) {  // the row-of-8 kernel's kinds (K8Kind)
+                const bool lane = (o.flags & FL_TARGET_PX) || o.cm_ext != 0;
+                switch (o.kind) {
+                    case F_DENSE:
+                    case F_SWAP: d.kind = lane ? K8_G2 : K8_U2; break;
+                    case F_REAL: d.kind = lane ? K8_G2 : K8_R2; break;
+                    case F_DIAG: d.kind = K8_DG; break;
+                    case F_PHASE:
+                        d.kind = K8_DG;
+                        d.g = -1 - o.g;
+                        break;
+                    case F_FLUSH: d.kind = K8_FL; break;
+                    default: throw Error(QFVM_ERR_RUNTIME, "op kind not supported by the row-of-8 kernel");
+                }
             }
             h.push_back(d);
         }
@@ -1301,6 +1654,22 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_fused_pass<R, NS>), s.device);
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
+        if (it.pass.group_bits == 3) {
+            if constexpr (NS == 1) {
+                // the load swizzle needs at least 128 threads (slot bits 0-6 from the thread index)
+                if (pd.b < 10 || pd.b > 13) throw Error(QFVM_ERR_RUNTIME, "row-of-8 tiles hold 2^10 to 2^13 amplitudes");
+                const size_t lds8 = ((size_t)1 << pd.b) * sizeof(V2<R>) + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
+                if (lds8 > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "row-of-8 tile exceeds 160 KiB of LDS");
+                if (lds8 > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_pass8<R>), s.device);
+                KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
+                hipLaunchKernelGGL(k_pass8<R>, dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds8, s.stream, psi, pd, ops,
+                                   druns);
+                QFVM_LAUNCHED();
+                continue;
+            } else {
+                throw Error(QFVM_ERR_RUNTIME, "row-of-8 passes are one-state passes");
+            }
+        }
         // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
         // between passes stages its matrix there)
         double* partials = cols > 0 ? reinterpret_cast<double*>(s.ensure_scratch(cols * tiles * sizeof(double))) : nullptr;

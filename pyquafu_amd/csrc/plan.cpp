@@ -139,6 +139,7 @@ void mark_diagonal_runs(FGroup& fg) {
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg) {
     const int b = std::min(cfg.tile_bits, n);
+    const int gb = cfg.group_bits;
     // a tile keeps at least one free slot when it does not hold every qubit, so the first waiting
     // gate always fits and every pass makes progress
     const int L = b < n ? std::min(cfg.low_bits, b - 1) : b;
@@ -174,7 +175,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                 for (int j = (int)groups.size() - 1; j >= 0; j--) {
                     const GroupB& G = groups[j];
                     const bool fits = g.grad ? G.stage
-                                             : (!G.stage && ((G.S & g.N) == g.N || popc(G.S | g.N) <= FUSE_GROUP_BITS));
+                                             : (!G.stage && ((G.S & g.N) == g.N || popc(G.S | g.N) <= gb));
                     if (fits) {
                         place = j;
                         break;
@@ -214,9 +215,10 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             Q = newQ;
             done[i] = 1;
         }
-        // finalize: pad the tile to b qubits and every group to 4 tile qubits
+        // finalize: pad the tile to b qubits and every group to gb tile qubits
         for (int q = 0; q < n && popc(Q) < b; q++) Q |= 1ull << q;
         PlanItem it{true, FPass{}, Gate{}};
+        it.pass.group_bits = gb;
         for (int q = 0; q < n; q++)
             if ((Q >> q) & 1) it.pass.Q.push_back(q);
         for (GroupB& G : groups) {
@@ -237,7 +239,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                 continue;
             }
             for (int q : it.pass.Q) {
-                if (popc(G.S) >= FUSE_GROUP_BITS) break;
+                if (popc(G.S) >= gb) break;
                 G.S |= 1ull << q;
             }
             FGroup fg;
@@ -299,6 +301,9 @@ static FuseConfig config_of(const State& s, int states) {
     cfg.low_bits = s.low_bits > 0 ? s.low_bits : default_low_bits(s.dtype, states);
     cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
     cfg.max_cost = s.pass_cost;
+    // complex128 one-state passes run on the row-of-8 kernel (3-qubit groups) when its tile fits
+    const int b = std::min(cfg.tile_bits, s.n);
+    if (states == 1 && s.dtype == QFVM_C128 && s.rows8 && b >= 10 && b <= 13) cfg.group_bits = 3;
     return cfg;
 }
 

@@ -36,16 +36,19 @@ def replay(plan, prog, n, psi):
             continue
         Q = it["Q"]
         assert Q == sorted(Q) and len(set(Q)) == len(Q)
+        gb = it["group_bits"]
+        assert gb in (3, 4)
         for G in it["groups"]:
-            S = G["S"]
-            assert len(set(S)) == 4 and set(S) <= set(Q)
+            S = G["S"][:gb]
+            assert len(set(S)) == gb and set(S) <= set(Q) and all(x == -1 for x in G["S"][gb:])
             for op in G["ops"]:
                 if op["kind"] == F_FLUSH:  # moves data between registers only; no logical effect
-                    assert 0 <= op["g"] < 4
+                    assert 0 <= op["g"] < gb
                     continue
                 m = [complex(*z) for z in op["m"]]
                 cm = int(op["cme"])
-                for j in range(4):
+                assert op["cmg"] < (1 << gb)
+                for j in range(gb):
                     if (op["cmg"] >> j) & 1:
                         cm |= 1 << S[j]
                 if op["kind"] == F_PHASE:
@@ -87,7 +90,7 @@ def check(prog, n, **kw):
 
 
 @pytest.mark.parametrize("n,tile,low,groups", [(6, 0, 0, 0), (9, 6, 2, 2), (10, 8, 4, 3), (12, 0, 0, 0),
-                                                (13, 7, 3, 12), (14, 12, 6, 24)])
+                                                (13, 7, 3, 12), (14, 12, 6, 24), (16, 9, 4, 0), (17, 13, 5, 6)])
 def test_random_programs_replay(n, tile, low, groups):
     rng = np.random.Generator(np.random.PCG64(500 + n))
     prog = random_gate_program(n, 300, rng, max_multi=3)
