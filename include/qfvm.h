@@ -266,6 +266,8 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 two-state passes of the adjoint (both precisions, 2^10 to 2^12 per state) run
  *                 with two lanes per 16-amplitude register row (twice the wavefronts per tile),
  *                 0 = one lane per row
+ *   "rows8"       1 = (default) complex128 one-state passes of 2^10..2^13-amplitude tiles hold one
+ *                 8-amplitude row (3-qubit groups) per lane instead of split rows, 0 = split rows
  *   "stream"      1 = persistent fused-pass kernel that prefetches the next tile (complex128,
  *                 one-state passes of 2^11..2^13-amplitude tiles, when split_rows does not apply),
  *                 0 = one workgroup per tile

@@ -27,6 +27,10 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 # (gate kind, target bit) case; SimplifyCFG's sinking of identical-looking cases would merge them
 # into one block with a runtime index and push the array to scratch memory.
 DEVICE += ["-mllvm", "-simplifycfg-sink-common=false"]
+# The fused kernels dispatch on op records through nested switches whose conditions are uniform
+# (scalar registers); structurizing those regions into predicated flow blocks costs register moves
+# at every join (measured on the row-of-8 kernel: 112 instead of 122 VGPRs, 30% fewer moves).
+DEVICE += ["-mllvm", "-structurizecfg-skip-uniform-regions"]
 
 SOURCES = ["gates.hip", "reduce.hip", "sample.hip", "fused.hip", "plan.cpp", "capi.cpp"]
 

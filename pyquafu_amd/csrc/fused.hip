@@ -42,6 +42,12 @@ struct PassDesc {
     uint32_t gpxc[FUSE_MAX_GROUPS];                   // relabel constants at each group's end
     uint64_t top[8];     // global offsets of local index j << (b - 3): the tile's 3 highest local bits
     uint64_t top16[16];  // the same for j << (b - 4): the 4 highest
+    // row-of-8 passes: byte offsets swz(member m's group bits) * 16 of each group, the uniform
+    // relabel offset of groups without lane parities, and per-group flags (bit gi): the group needs
+    // the lane's global index (lane-dependent ops or relabel parities) / its relabeling is per lane
+    int32_t goff[FUSE_MAX_GROUPS][8];
+    int32_t gpxoff[FUSE_MAX_GROUPS];
+    uint32_t g0_mask, pxlane_mask;
 };
 
 // Diagonal-run record (fused.h): 16 host factors f0, then for each subset S its term range
@@ -993,15 +999,16 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
 
 // ------------------------------------------------------------------------------------------------
 // Row-of-8 pass kernel (complex128 one-state passes of 2^10 .. 2^13-amplitude tiles, 3-qubit
-// groups). One lane holds the 8
-// amplitudes that differ only in its group's 3 bits, so no gate needs another lane's data (the
-// split-row kernel pairs lanes for its 4th group bit, and each of those gates pays 32 DPP moves);
-// a 2^b tile still gets 2^(b-3) threads, and the load / store skeleton is the split-row kernel's.
+// groups). One lane holds the 8 amplitudes that differ only in its group's 3 bits, so no gate
+// needs another lane's data (the split-row kernel pairs lanes for its 4th group bit, and each of
+// those gates pays 32 DPP moves); a 2^b tile still gets 2^(b-3) threads, and the load / store
+// skeleton is the split-row kernel's.
 // The op loop alternates two register sets (a -> o, then o -> a), so no op copies its results
 // back, and the common ops -- a 2x2 on one group bit whose coefficients are the same for every
-// lane -- keep their coefficients in scalar registers: measured on the 30-qubit shape, 12 to 18
-// such gates per pass cost almost nothing over the bare load / store (the arithmetic overlaps
-// the HBM traffic). Device op kinds (FOpDev.kind):
+// lane -- keep their coefficients in scalar registers. The pass is bound by HBM only while the
+// vector instructions per wave stay well under the load / store time, so the skeleton is kept
+// lean: member slots are XORs of host-computed offsets and the lane's global index is built only
+// in groups whose ops or relabeling depend on it. Device op kinds (FOpDev.kind):
 enum K8Kind : int32_t {
     K8_U2 = 0,  // complex 2x2 on group bit g, the same for every lane; pairs outside the group
                 //   controls cm_group keep their values
@@ -1011,12 +1018,13 @@ enum K8Kind : int32_t {
     K8_DG = 3,  // diagonal: diag(d0, d1) on group bit g, or (g = -1 - q) on qubit q outside the
                 //   group, with group / outside controls and relabel swaps
     K8_FL = 4,  // relabel flush of group bit g (pairs swap where the lane's parity is set)
-    K8_DR = 5   // diagonal run (the split-row kernel's records, subsets of 3 bits)
+    K8_DR = 5,  // diagonal run (the split-row kernel's records, subsets of 3 bits)
+    K8_SW = 6   // exchange of the pair members on group bit g where the group controls are set
+                //   (X and controlled X inside the group): register moves only
 };
 
-// An op record as the row-of-8 kernel holds it: the header (kind, g, cm_group, flags), the
-// outside-control and relabel masks and the 4 coefficients, fetched with wide scalar loads so
-// that the next op's record can be in flight while the current op computes.
+// An op record as the row-of-8 kernel reads it (wide scalar loads of the header, the outside-
+// control and relabel masks and the 4 coefficients)
 template <typename R>
 struct Op8 {
     int32_t kind, g, cm, flags;
@@ -1040,18 +1048,22 @@ __device__ __forceinline__ Op8<R> load_op8(const FOpDev<R>* __restrict__ p) {
     return o;
 }
 
-// 2x2 on group bit G for the pairs whose group-control bits CM are set (CM a compile-time subset
-// of the other two bits); the other pairs are copied
-template <typename R, int G, int CM, bool REAL, int M>
+// Pair (M, M | 1 << G) of a gate on group bit G, for the pairs whose group-control bits CM are
+// set (CM a compile-time subset of the other two bits); the other pairs are copied. OPK: 0 complex
+// 2x2, 1 real 2x2, 2 exchange.
+template <typename R, int G, int CM, int OPK, int M>
 __device__ __forceinline__ void r8_pair(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11) {
     if constexpr (!((M >> G) & 1)) {
         constexpr int M1 = M | (1 << G);
         if constexpr ((M & CM) != CM) {
             o[M] = a[M];
             o[M1] = a[M1];
+        } else if constexpr (OPK == 2) {
+            o[M] = a[M1];
+            o[M1] = a[M];
         } else {
             const V2<R> x = a[M], y = a[M1];
-            if constexpr (REAL) {
+            if constexpr (OPK == 1) {
                 o[M] = mk<R>(m00.x * x.x + m01.x * y.x, m00.x * x.y + m01.x * y.y);
                 o[M1] = mk<R>(m10.x * x.x + m11.x * y.x, m10.x * x.y + m11.x * y.y);
             } else {
@@ -1062,35 +1074,35 @@ __device__ __forceinline__ void r8_pair(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R
     }
 }
 
-template <typename R, int G, int CM, bool REAL, int... Ms>
+template <typename R, int G, int CM, int OPK, int... Ms>
 __device__ __forceinline__ void r8_2x2(const V2<R> (&a)[8], V2<R> (&o)[8], V2<R> m00, V2<R> m01, V2<R> m10, V2<R> m11,
                                        std::integer_sequence<int, Ms...>) {
-    (r8_pair<R, G, CM, REAL, Ms>(a, o, m00, m01, m10, m11), ...);
+    (r8_pair<R, G, CM, OPK, Ms>(a, o, m00, m01, m10, m11), ...);
 }
 
 // the group-control masks of a gate on bit G: subsets of the other two bits
-template <typename R, int G, bool REAL>
+template <typename R, int G, int OPK>
 __device__ __forceinline__ void r8_ctrl(const V2<R> (&a)[8], V2<R> (&o)[8], int cm, V2<R> m00, V2<R> m01, V2<R> m10,
                                         V2<R> m11) {
     constexpr int B0 = G == 0 ? 2 : 1, B1 = G == 2 ? 2 : 4;
     constexpr auto seq = std::make_integer_sequence<int, 8>{};
     if (cm == 0)
-        r8_2x2<R, G, 0, REAL>(a, o, m00, m01, m10, m11, seq);
+        r8_2x2<R, G, 0, OPK>(a, o, m00, m01, m10, m11, seq);
     else if (cm == B0)
-        r8_2x2<R, G, B0, REAL>(a, o, m00, m01, m10, m11, seq);
+        r8_2x2<R, G, B0, OPK>(a, o, m00, m01, m10, m11, seq);
     else if (cm == B1)
-        r8_2x2<R, G, B1, REAL>(a, o, m00, m01, m10, m11, seq);
+        r8_2x2<R, G, B1, OPK>(a, o, m00, m01, m10, m11, seq);
     else
-        r8_2x2<R, G, B0 | B1, REAL>(a, o, m00, m01, m10, m11, seq);
+        r8_2x2<R, G, B0 | B1, OPK>(a, o, m00, m01, m10, m11, seq);
 }
 
-template <typename R, bool REAL>
+template <typename R, int OPK>
 __device__ __forceinline__ void r8_bit(const V2<R> (&a)[8], V2<R> (&o)[8], int g, int cm, V2<R> m00, V2<R> m01, V2<R> m10,
                                        V2<R> m11) {
     switch (g) {
-        case 0: r8_ctrl<R, 0, REAL>(a, o, cm, m00, m01, m10, m11); break;
-        case 1: r8_ctrl<R, 1, REAL>(a, o, cm, m00, m01, m10, m11); break;
-        default: r8_ctrl<R, 2, REAL>(a, o, cm, m00, m01, m10, m11); break;
+        case 0: r8_ctrl<R, 0, OPK>(a, o, cm, m00, m01, m10, m11); break;
+        case 1: r8_ctrl<R, 1, OPK>(a, o, cm, m00, m01, m10, m11); break;
+        default: r8_ctrl<R, 2, OPK>(a, o, cm, m00, m01, m10, m11); break;
     }
 }
 
@@ -1191,13 +1203,17 @@ __device__ __forceinline__ void r8_op(const V2<R> (&a)[8], V2<R> (&o)[8], const 
                                       const unsigned char* __restrict__ runs) {
     switch (op.kind) {
         case K8_U2:
-            r8_bit<R, false>(a, o, op.g, op.cm, mk<R>(op.m[0], op.m[1]), mk<R>(op.m[2], op.m[3]), mk<R>(op.m[4], op.m[5]),
-                             mk<R>(op.m[6], op.m[7]));
+            r8_bit<R, 0>(a, o, op.g, op.cm, mk<R>(op.m[0], op.m[1]), mk<R>(op.m[2], op.m[3]), mk<R>(op.m[4], op.m[5]),
+                         mk<R>(op.m[6], op.m[7]));
             break;
         case K8_R2:
-            r8_bit<R, true>(a, o, op.g, op.cm, mk<R>(op.m[0], 0), mk<R>(op.m[2], 0), mk<R>(op.m[4], 0),
-                            mk<R>(op.m[6], 0));
+            r8_bit<R, 1>(a, o, op.g, op.cm, mk<R>(op.m[0], 0), mk<R>(op.m[2], 0), mk<R>(op.m[4], 0), mk<R>(op.m[6], 0));
             break;
+        case K8_SW: {
+            const V2<R> z = mk<R>(0, 0);
+            r8_bit<R, 2>(a, o, op.g, op.cm, z, z, z, z);
+            break;
+        }
         case K8_G2: {
             V2<R> m00 = mk<R>(op.m[0], op.m[1]), m01 = mk<R>(op.m[2], op.m[3]);
             V2<R> m10 = mk<R>(op.m[4], op.m[5]), m11 = mk<R>(op.m[6], op.m[7]);
@@ -1209,12 +1225,14 @@ __device__ __forceinline__ void r8_op(const V2<R> (&a)[8], V2<R> (&o)[8], const 
                 m10 = csel<R>(sw, t01, t10);
                 m11 = csel<R>(sw, t00, t11);
             }
-            const bool ok = (G0 & op.cm_ext) == op.cm_ext;
-            m00 = csel<R>(ok, m00, mk<R>(1, 0));
-            m01 = csel<R>(ok, m01, mk<R>(0, 0));
-            m10 = csel<R>(ok, m10, mk<R>(0, 0));
-            m11 = csel<R>(ok, m11, mk<R>(1, 0));
-            r8_bit<R, false>(a, o, op.g, op.cm, m00, m01, m10, m11);
+            if (op.cm_ext) {
+                const bool ok = (G0 & op.cm_ext) == op.cm_ext;
+                m00 = csel<R>(ok, m00, mk<R>(1, 0));
+                m01 = csel<R>(ok, m01, mk<R>(0, 0));
+                m10 = csel<R>(ok, m10, mk<R>(0, 0));
+                m11 = csel<R>(ok, m11, mk<R>(1, 0));
+            }
+            r8_bit<R, 0>(a, o, op.g, op.cm, m00, m01, m10, m11);
             break;
         }
         case K8_DG: {
@@ -1247,78 +1265,97 @@ __device__ __forceinline__ void r8_op(const V2<R> (&a)[8], V2<R> (&o)[8], const 
             }
             break;
         }
-        default: r8_run<R>(a, o, runs, op, G0, std::make_integer_sequence<int, 8>{}); break;
+        default:
+            r8_run<R>(a, o, runs, op, G0, std::make_integer_sequence<int, 8>{});
+            break;
     }
 }
 
+// global offset of a local index: identity below L, then bit k -> qubit q[k] (branch-free)
+__device__ __forceinline__ uint64_t local_to_global_bits(int l, const PassDesc& pd) {
+    uint64_t g = (uint64_t)(l & ((1 << pd.L) - 1));
+    for (int k = pd.L; k < pd.b; k++) g |= (uint64_t)((l >> k) & 1) << pd.q[k];
+    return g;
+}
+
+// Direct HBM -> LDS loads of the tile at global offset `base` (as k_fused_half): slot lh + tid
+// receives element lh + swz(tid), the 8 loads cover the tile's 3 highest local bits. lo_sw is the
+// thread's offset local_to_global_low(swz(tid)).
+template <typename R>
+__device__ __forceinline__ void pass8_load(const V2<R>* __restrict__ psi, V2<R>* tile, uint64_t base, uint64_t lo_sw,
+                                           const PassDesc& pd, int NT, int wave) {
+    __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const void* src = (const void*)(psi + (base | lo_sw | pd.top[j]));
+        auto* dst = (__attribute__((address_space(3))) void*)(tile + j * NT + wave * 64);
+        __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
+    }
+    __builtin_amdgcn_s_setprio(0);
+}
+
+// The pass's groups on the tile in LDS (at byte address `tb`): each group gathers its 8 members,
+// runs its ops and scatters them back under its relabeling; every group ends at a barrier.
+template <typename R>
+__device__ __forceinline__ void pass8_groups(unsigned char* tb, int tid, uint64_t base, const PassDesc& pd,
+                                             const FOpDev<R>* __restrict__ ops, const unsigned char* __restrict__ runs) {
+    for (int gi = 0; gi < pd.ngroups; gi++) {
+        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2];
+        const int l0 = ins_bit(ins_bit(ins_bit(tid, s0), s1), s2);
+        // slot of member m = swz(l0 | bits of m) = swz(l0) ^ swz(bits of m) (the swizzle is linear
+        // over XOR): one XOR with a host-computed byte offset per member
+        const int sl = swz(l0) << 4;
+        V2<R> A[8], B[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) A[m] = *reinterpret_cast<const V2<R>*>(tb + (sl ^ pd.goff[gi][m]));
+        const uint64_t G0 = ((pd.g0_mask >> gi) & 1u) ? base | local_to_global_bits(l0, pd) : 0;
+        // two ops per iteration, a -> b and b -> a
+        const int o_end = pd.gop[gi + 1];
+        int oi = pd.gop[gi];
+        if (oi < o_end) {
+            for (;;) {
+                r8_op<R>(A, B, load_op8<R>(ops + oi), G0, runs);
+                if (++oi >= o_end) {
+#pragma unroll
+                    for (int m = 0; m < 8; m++) A[m] = B[m];
+                    break;
+                }
+                r8_op<R>(B, A, load_op8<R>(ops + oi), G0, runs);
+                if (++oi >= o_end) break;
+            }
+        }
+        // the group's relabeling, applied by the scatter (register m goes to logical m ^ px)
+        int sx = sl ^ pd.gpxoff[gi];
+        if ((pd.pxlane_mask >> gi) & 1u) {
+            const uint32_t pc = pd.gpxc[gi];
+            int px = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const uint64_t msk = pd.gpxm[gi][j];
+                const int bit = msk ? px_bit(G0, msk, pc >> j) : (int)((pc >> j) & 1u);
+                px |= bit << pd.gbits[gi][j];
+            }
+            sx = sl ^ (swz(px) << 4);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) *reinterpret_cast<V2<R>*>(tb + (sx ^ pd.goff[gi][m])) = A[m];
+        __syncthreads();
+    }
+}
+
+// One tile per workgroup.
 template <typename R>
 __global__ void __launch_bounds__(1024) k_pass8(V2<R>* __restrict__ psi, const PassDesc pd,
                                                 const FOpDev<R>* __restrict__ ops, const unsigned char* __restrict__ runs) {
     extern __shared__ unsigned char lds_raw[];
     V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
-    const int T = 1 << pd.b;
-    const int tid = threadIdx.x, NT = blockDim.x;  // NT = T / 8: one row of 8 per thread
+    const int tid = threadIdx.x, NT = blockDim.x;  // NT = 2^b / 8: one row of 8 per thread
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint64_t base = tile_base(blockIdx.x, pd);
-    uint64_t* hi_tab = reinterpret_cast<uint64_t*>(tile + T);  // global offsets of the high local bits
-    const int L = pd.L;
-    {
-        // as k_fused_half: slot lh + tid receives element lh + swz(tid) (direct HBM -> LDS loads),
-        // the 8 loads cover the tile's 3 highest local bits, issued at a raised wave priority
-        __builtin_amdgcn_s_setprio(2);
-        const uint64_t g_sw = base | local_to_global_low(swz(tid), pd);
-        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const void* src = (const void*)(psi + (g_sw | pd.top[j]));
-            auto* dst = (__attribute__((address_space(3))) void*)(tile + j * NT + wave * 64);
-            __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
-        }
-        __builtin_amdgcn_s_setprio(0);
-    }
-    for (int i = tid; i < (T >> L); i += NT) hi_tab[i] = local_to_global(i << L, pd);
+    pass8_load<R>(psi, tile, base, local_to_global_low(swz(tid), pd), pd, NT, wave);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int gi = 0; gi < pd.ngroups; gi++) {
-        const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2];
-        const int l0 = ins_bit(ins_bit(ins_bit(tid, s0), s1), s2);
-        const int sl = swz(l0);
-        const uint64_t G0 = base | (uint64_t)(l0 & ((1 << L) - 1)) | hi_tab[l0 >> L];
-        V2<R> A[8], B[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) A[m] = tile[sl ^ swz(((m & 1) << s0) | (((m >> 1) & 1) << s1) | (((m >> 2) & 1) << s2))];
-        int oi = pd.gop[gi];
-        const int o_end = pd.gop[gi + 1];
-        if (oi < o_end) {
-            // two ops per iteration (a -> b, b -> a); the next records load while these compute
-            Op8<R> cur = load_op8<R>(ops + oi);
-            for (;;) {
-                const Op8<R> nxt = load_op8<R>(ops + min(oi + 1, o_end - 1));
-                r8_op<R>(A, B, cur, G0, runs);
-                if (oi + 1 >= o_end) {
-#pragma unroll
-                    for (int m = 0; m < 8; m++) A[m] = B[m];
-                    break;
-                }
-                cur = load_op8<R>(ops + min(oi + 2, o_end - 1));
-                r8_op<R>(B, A, nxt, G0, runs);
-                oi += 2;
-                if (oi >= o_end) break;
-            }
-        }
-        // the group's relabeling, applied by the scatter (register m goes to logical m ^ px)
-        const uint32_t pc = pd.gpxc[gi];
-        int px = 0;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const uint64_t msk = pd.gpxm[gi][j];
-            const int bit = msk ? px_bit(G0, msk, pc >> j) : (int)((pc >> j) & 1u);
-            px |= bit << pd.gbits[gi][j];
-        }
-        const int sx = sl ^ swz(px);
-#pragma unroll
-        for (int m = 0; m < 8; m++) tile[sx ^ swz(((m & 1) << s0) | (((m >> 1) & 1) << s1) | (((m >> 2) & 1) << s2))] = A[m];
-        __syncthreads();
-    }
+    pass8_groups<R>(lds_raw, tid, base, pd, ops, runs);
     const uint64_t g_tid = base | local_to_global_low(tid, pd);
     __builtin_amdgcn_s_setprio(2);
 #pragma unroll
@@ -1514,13 +1551,36 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
         pd.gop[gi] = (int)h.size() - first;
         for (int j = 0; j < FUSE_GROUP_BITS; j++) pd.gpxm[gi][j] = j < gbn ? G.pxm[j] : 0;
         pd.gpxc[gi] = G.pxc;
+        if (gbn == 3) {  // member slot offsets and the relabeling of the row-of-8 kernel
+            const auto hswz = [](int l) { return l ^ ((l >> 4) & 7); };
+            for (int m = 0; m < 8; m++) {
+                int c = 0;
+                for (int j = 0; j < 3; j++)
+                    if ((m >> j) & 1) c |= 1 << pd.gbits[gi][j];
+                pd.goff[gi][m] = hswz(c) << 4;
+            }
+            int pxc = 0;
+            bool lane_px = false;
+            for (int j = 0; j < 3; j++) {
+                if (G.pxm[j]) lane_px = true;
+                if ((G.pxc >> j) & 1u) pxc |= 1 << pd.gbits[gi][j];
+            }
+            pd.gpxoff[gi] = hswz(pxc) << 4;
+            if (lane_px) {
+                pd.pxlane_mask |= 1u << gi;
+                pd.g0_mask |= 1u << gi;
+            }
+        }
         int done_run = 0;
         for (const FOpHost& o : G.ops) {
             if (o.kind == F_RELABEL) continue;  // folded into the parity masks
             if (o.run > 0) {                    // runs are numbered in op order
                 if (o.run > done_run) {
                     h.push_back(encode_run<R>(G, o.run, runs));
-                    if (gbn == 3) h.back().kind = K8_DR;
+                    if (gbn == 3) {
+                        h.back().kind = K8_DR;
+                        pd.g0_mask |= 1u << gi;
+                    }
                     done_run = o.run;
                 }
                 continue;
@@ -1539,8 +1599,8 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
             if (gbn == 3) {  // the row-of-8 kernel's kinds (K8Kind)
                 const bool lane = (o.flags & FL_TARGET_PX) || o.cm_ext != 0;
                 switch (o.kind) {
-                    case F_DENSE:
-                    case F_SWAP: d.kind = lane ? K8_G2 : K8_U2; break;
+                    case F_DENSE: d.kind = lane ? K8_G2 : K8_U2; break;
+                    case F_SWAP: d.kind = lane ? K8_G2 : K8_SW; break;
                     case F_REAL: d.kind = lane ? K8_G2 : K8_R2; break;
                     case F_DIAG: d.kind = K8_DG; break;
                     case F_PHASE:
@@ -1550,6 +1610,8 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
                     case F_FLUSH: d.kind = K8_FL; break;
                     default: throw Error(QFVM_ERR_RUNTIME, "op kind not supported by the row-of-8 kernel");
                 }
+                // ops that read the lane's global index
+                if (lane || o.kind == F_PHASE || o.kind == F_FLUSH) pd.g0_mask |= 1u << gi;
             }
             h.push_back(d);
         }
@@ -1571,22 +1633,26 @@ void allow_large_lds(const void* fn, int device) {
 }
 
 // Workgroups of a persistent k_fused_stream launch: as many as fit on the device at once.
-static unsigned stream_grid(const State& s, size_t lds, unsigned threads, uint64_t tiles) {
-    static std::vector<std::pair<uint64_t, unsigned>> known;  // (device, lds, threads) -> grid
-    static std::mutex known_lock;                              // handles run on several host threads
+static unsigned stream_grid(const State& s, const void* fn, size_t lds, unsigned threads, uint64_t tiles) {
+    struct Known {
+        const void* fn;
+        uint64_t key;
+        unsigned grid;
+    };
+    static std::vector<Known> known;  // (kernel, device, lds, threads) -> grid
+    static std::mutex known_lock;     // handles run on several host threads
     std::lock_guard<std::mutex> guard(known_lock);
     const uint64_t key = ((uint64_t)s.device << 40) | ((uint64_t)lds << 12) | threads;
     unsigned grid = 0;
     for (const auto& kv : known)
-        if (kv.first == key) grid = kv.second;
+        if (kv.fn == fn && kv.key == key) grid = kv.grid;
     if (grid == 0) {
-        allow_large_lds(reinterpret_cast<const void*>(&k_fused_stream), s.device);
+        allow_large_lds(fn, s.device);
         int per_cu = 0, cus = 0;
-        QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused_stream),
-                                                              (int)threads, lds));
+        QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds));
         QFVM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device));
         grid = (unsigned)std::max(1, per_cu * cus);
-        known.emplace_back(key, grid);
+        known.push_back({fn, key, grid});
     }
     return (unsigned)std::min<uint64_t>(grid, tiles);
 }
@@ -1655,19 +1721,19 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
         if (it.pass.group_bits == 3) {
-            if constexpr (NS == 1) {
+            if constexpr (NS == 1 && sizeof(R) == 8) {
                 // the load swizzle needs at least 128 threads (slot bits 0-6 from the thread index)
                 if (pd.b < 10 || pd.b > 13) throw Error(QFVM_ERR_RUNTIME, "row-of-8 tiles hold 2^10 to 2^13 amplitudes");
-                const size_t lds8 = ((size_t)1 << pd.b) * sizeof(V2<R>) + ((size_t)1 << (pd.b - pd.L)) * sizeof(uint64_t);
+                const size_t lds8 = ((size_t)1 << pd.b) * sizeof(V2<R>);
+                const unsigned threads8 = (1u << pd.b) >> 3;
+                KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
                 if (lds8 > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "row-of-8 tile exceeds 160 KiB of LDS");
                 if (lds8 > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_pass8<R>), s.device);
-                KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
-                hipLaunchKernelGGL(k_pass8<R>, dim3((unsigned)tiles), dim3((1u << pd.b) >> 3), lds8, s.stream, psi, pd, ops,
-                                   druns);
+                hipLaunchKernelGGL(k_pass8<R>, dim3((unsigned)tiles), dim3(threads8), lds8, s.stream, psi, pd, ops, druns);
                 QFVM_LAUNCHED();
                 continue;
             } else {
-                throw Error(QFVM_ERR_RUNTIME, "row-of-8 passes are one-state passes");
+                throw Error(QFVM_ERR_RUNTIME, "row-of-8 passes are one-state complex128 passes");
             }
         }
         // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate
@@ -1693,7 +1759,7 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         }
         if constexpr (NS == 1 && sizeof(R) == 8) {
             if (s.stream_tiles && pd.b >= 11 && pd.b <= 13 && pd.ngroups >= 1 && pd.ngroups <= FUSE_MAX_GROUPS) {
-                const unsigned grid = stream_grid(s, lds, threads, tiles);
+                const unsigned grid = stream_grid(s, reinterpret_cast<const void*>(&k_fused_stream), lds, threads, tiles);
                 KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
                 hipLaunchKernelGGL(k_fused_stream, dim3(grid), dim3(threads), lds, s.stream,
                                    reinterpret_cast<double2*>(psi), pd, reinterpret_cast<const FOpDev<double>*>(ops),
