@@ -32,6 +32,11 @@ import subprocess
 import sys
 import time
 
+# The batch leg runs circuits on 8 engine streams; HIP maps a process's streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 by default), and kernels that share a queue run one after
+# another. Read once at HIP initialisation, so it is set before anything touches the GPU.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -292,12 +297,15 @@ def qft_leg(n: int, shots: int, samples: int = 100000):
     return out
 
 
-def batch_leg(count: int, workers: int, graphs: bool = False):
+def batch_leg(count: int, workers: int, graphs: bool = True, steps: int = 3, pin_qubits: int = 21):
     """Batched uneven workload: `count` seeded circuits (PCG64 seed 20250004), n ~ U{10..22}, depth ~
     U{4..16} layers of RY and RZ on every qubit plus a CZ ring, complex64; each evaluates <sum_i Z_i>
-    and its full adjoint gradient. `workers` host threads drive one engine handle (one HIP stream)
-    per qubit count each; tile shapes from the autotuner. Compilation and tuning happen before
-    the timed run; one untimed run warms the plans."""
+    and its full adjoint gradient. One engine call per step (qfvm_batch_run): `workers` engine
+    threads, one handle (one HIP stream) per qubit count each; tile shapes and fusion width from the
+    autotuner. Compilation, tuning and one untimed step (plans, graphs) come first; then `steps`
+    training steps, each rebinding every circuit's parameters in place (new values) and running the
+    batch. Reported: circuits/s and latencies of the steps, the host time of one rebinding of all
+    circuits against the engine call's wall time, and the same with graphs off."""
     from pyquafu_amd import workloads
     from pyquafu_amd.batch import Autotuner, BatchRunner, compile_item
 
@@ -306,45 +314,57 @@ def batch_leg(count: int, workers: int, graphs: bool = False):
     items = [compile_item(qc, workloads.z_sum(qc.num)) for qc, _ in circuits]
     t_compile = time.perf_counter() - t0
     tuner = Autotuner()
-    runner = BatchRunner(dtype="c64", workers=workers, tuner=tuner)
-    t0 = time.perf_counter()
-    runner.warm([it.n for it in items])
-    t_warm = time.perf_counter() - t0
-    runner.run(items)
-    results, wall = runner.run(items)
-    runner.close()
-    lat = np.array([r.latency_s for r in results]) * 1e3
-    graph_mode = None
-    if graphs:
-        # the same batch with every call captured into a hipGraph (option "graphs"): the first run
-        # instantiates one executable graph per circuit and handle, the timed run recaptures each
-        # call and updates its cached graph (the path a training step with new parameters takes)
-        grunner = BatchRunner(dtype="c64", workers=workers, tuner=tuner, graphs=True)
-        grunner.warm([it.n for it in items])
-        grunner.run(items)
-        gresults, gwall = grunner.run(items)
-        grunner.close()
-        glat = np.array([r.latency_s for r in gresults]) * 1e3
-        graph_mode = {
-            "circuits_per_s": round(count / gwall, 1),
-            "wall_ms": round(gwall * 1e3, 2),
-            "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
-            "latency_ms_p99": round(float(np.percentile(glat, 99)), 3),
-            "max_abs_diff_vs_eager": max(float(np.abs(a.grads - b.grads).max(initial=0.0))
-                                         for a, b in zip(gresults, results)),
-        }
+    rng = np.random.Generator(np.random.PCG64(20250006))
+    step_values = [[rng.uniform(-np.pi, np.pi, size=it.pp.n_vars) for it in items] for _ in range(steps)]
+
+    def leg(graph_mode: bool):
+        runner = BatchRunner(dtype="c64", workers=workers, tuner=tuner, graphs=graph_mode, pin_qubits=pin_qubits)
+        t0 = time.perf_counter()
+        runner.warm([it.n for it in items])
+        t_warm = time.perf_counter() - t0
+        for it, (_, v) in zip(items, circuits):
+            it.bind(v)
+        runner.run(items)  # untimed: plans, graphs
+        walls, binds, lats = [], [], []
+        for vals in step_values:
+            t0 = time.perf_counter()
+            for it, v in zip(items, vals):
+                it.bind(v)
+            binds.append(time.perf_counter() - t0)
+            results, wall = runner.run(items)
+            walls.append(wall)
+            lats.extend(r.latency_s for r in results)
+        runner.close()
+        lat = np.array(lats) * 1e3
+        wall = float(np.median(walls))
+        return {
+            "circuits_per_s": round(count / wall, 1),
+            "wall_ms": round(wall * 1e3, 2),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "bind_ms": round(float(np.median(binds)) * 1e3, 2),
+            "tune_s": round(t_warm, 2),
+        }, results
+
+    main, results = leg(graphs)
+    extra = {}
+    if graphs:  # the same steps eager (never graphs under a profiler: --no-graph-batch)
+        other, other_results = leg(False)
+        other["max_abs_grad_diff"] = max(float(np.abs(a.grads - b.grads).max(initial=0.0))
+                                         for a, b in zip(results, other_results))
+        extra["graphs_off"] = other
     return {
         "workload": f"{count} circuits, n 10-22, depth 4-16 (RY+RZ layers, CZ ring), c64, <sum Z> + adjoint gradient",
-        "circuits_per_s": round(count / wall, 1),
-        "wall_ms": round(wall * 1e3, 2),
-        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "graphs": bool(graphs),
+        **main,
+        "steps": steps,
         "workers": workers,
+        "pin_qubits": pin_qubits,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default)"),
         "parameters": int(sum(len(it.ops) for it in items)),
         "host_compile_s": round(t_compile, 2),
-        "tune_and_warm_s": round(t_warm, 2),
         "tiles": {str(k[0]): list(v) for k, v in sorted(tuner.choice.items())},
-        "graphs": graph_mode,
+        **extra,
     }
 
 
@@ -413,9 +433,11 @@ def main(argv=None) -> int:
     ap.add_argument("--no-dense", action="store_true", help="skip the dense k-target gate leg (matrix cores)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-circuit leg")
     ap.add_argument("--batch-count", type=int, default=256)
-    ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--batch-pin", type=int, default=21,
+                    help="circuits of at least this many qubits run one after another on one worker (0: off)")
     ap.add_argument("--no-graph-batch", action="store_true",
-                    help="skip the batch run with every engine call as a captured hipGraph (profiled runs skip it)")
+                    help="run the batch leg eagerly only, without hipGraph capture (profiled runs use this)")
     ap.add_argument("--qft-n", type=int, default=33)
     ap.add_argument("--qft-shots", type=int, default=1000000)
     args = ap.parse_args(argv)
@@ -534,7 +556,8 @@ def main(argv=None) -> int:
         out["adjoint"] = None
     if not args.no_batch:
         print(f"[bench] batch leg: {args.batch_count} circuits", file=sys.stderr, flush=True)
-        out["batch"] = batch_leg(args.batch_count, args.batch_workers, not args.no_graph_batch)
+        out["batch"] = batch_leg(args.batch_count, args.batch_workers, not args.no_graph_batch,
+                                 pin_qubits=args.batch_pin)
     else:
         out["batch"] = None
     if not args.no_qft:

@@ -200,6 +200,52 @@ int qfvm_adjoint(qfvm_handle h, const qfvm_program* prog, const int64_t* grad_op
                  double* expectation);
 
 /*
+ * Batched adjoint evaluation. No counterpart in the reference, which evaluates one circuit per call
+ * (Estimator.run, quafu/algorithms/estimator.py:186-229; the per-row loop of
+ * quafu/algorithms/interface/torch.py:76-79). A batch context keeps, per worker thread, one engine
+ * handle per qubit count (each with its own HIP stream), created on first use with the options set
+ * through qfvm_batch_set_option and kept between runs, so their plan and graph caches serve later
+ * runs of the same circuits with new parameter values.
+ */
+typedef struct qfvm_batch_ctx* qfvm_batch;
+
+typedef struct qfvm_batch_item {
+    const qfvm_program* prog; /* unitary program on n_qubits, evaluated from |0...0> */
+    int32_t n_qubits;
+    int32_t reserved;
+    const int64_t* grad_ops;  /* the differentiated ops and their generators, as qfvm_adjoint */
+    const char* grad_gen;
+    int64_t n_grads;
+    const char* chars;        /* the observable, as qfvm_adjoint */
+    const int32_t* pos;
+    const int64_t* off;
+    const int32_t* len;
+    const double* coef;
+    int64_t n_terms;
+    double* grads;            /* out: n_grads values */
+    double* expectation;      /* out: <H> */
+    double* seconds;          /* out, may be null: wall time of this item's evaluation */
+} qfvm_batch_item;
+
+int qfvm_batch_create(int32_t dtype, int32_t workers, qfvm_batch* out);
+int qfvm_batch_destroy(qfvm_batch b);
+/*
+ * qfvm_set_option on the batch's handles of n_qubits (0: all), those that exist and those created
+ * later. One key belongs to the batch itself: "pin_qubits" (0..34, default 20), the qubit count
+ * from which items all run on worker 0 one after another (circuits that fill the GPU alone only
+ * stretch each other's latency side by side); 0 = no pinning.
+ */
+int qfvm_batch_set_option(qfvm_batch b, int32_t n_qubits, const char* key, int64_t value);
+/*
+ * Evaluates every item as qfvm_init_zero + qfvm_adjoint would on a handle of its own. Items of at
+ * least pin_qubits qubits go to worker 0, the others largest first (cost 2^n x (ops + 8)) onto the
+ * least-loaded worker, so one item list always gets the same assignment (and a circuit evaluated
+ * again meets its cached plans and graphs).
+ * All workers finish; the first failure's status is returned (its message in qfvm_last_error).
+ */
+int qfvm_batch_run(qfvm_batch b, const qfvm_batch_item* items, int64_t n_items);
+
+/*
  * Amplitudes at the given indices (out: n re, im pairs, complex128). No counterpart in the
  * reference, where the caller holds the whole state in host memory; here it avoids copying a
  * state of up to 256 GiB to read a few amplitudes.
@@ -266,6 +312,10 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 two-state passes of the adjoint (both precisions, 2^10 to 2^12 per state) run
  *                 with two lanes per 16-amplitude register row (twice the wavefronts per tile),
  *                 0 = one lane per row
+ *   "plan_cache"  1 = (default) fused-pass plans are kept per handle (up to 64, least recently used
+ *                 dropped first) and reused for any later gate list of the same structure -- the
+ *                 same gate kinds on the same qubits, only parameter values changed -- with the new
+ *                 coefficients; 0 = plan every call (and drop the kept plans)
  *   "rows8"       1 = (default) complex128 one-state passes of 2^10..2^13-amplitude tiles hold one
  *                 8-amplitude row (3-qubit groups) per lane instead of split rows, 0 = split rows
  *   "stream"      1 = persistent fused-pass kernel that prefetches the next tile (complex128,

@@ -42,6 +42,28 @@ class CProgram(ctypes.Structure):
     ]
 
 
+class CBatchItem(ctypes.Structure):
+    """qfvm_batch_item (include/qfvm.h)."""
+
+    _fields_ = [
+        ("prog", ctypes.POINTER(CProgram)),
+        ("n_qubits", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("grad_ops", ctypes.c_void_p),
+        ("grad_gen", ctypes.c_void_p),
+        ("n_grads", ctypes.c_int64),
+        ("chars", ctypes.c_void_p),
+        ("pos", ctypes.c_void_p),
+        ("off", ctypes.c_void_p),
+        ("len", ctypes.c_void_p),
+        ("coef", ctypes.c_void_p),
+        ("n_terms", ctypes.c_int64),
+        ("grads", ctypes.c_void_p),
+        ("expectation", ctypes.c_void_p),
+        ("seconds", ctypes.c_void_p),
+    ]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -75,6 +97,10 @@ SIGNATURES = {
     "qfvm_set_profiling": (ctypes.c_int, [_vp, _i32]),
     "qfvm_profile_json": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64, ctypes.POINTER(_i64), _i32]),
     "qfvm_set_option": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64]),
+    "qfvm_batch_create": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_vp)]),
+    "qfvm_batch_destroy": (ctypes.c_int, [_vp]),
+    "qfvm_batch_set_option": (ctypes.c_int, [_vp, _i32, ctypes.c_char_p, _i64]),
+    "qfvm_batch_run": (ctypes.c_int, [_vp, ctypes.POINTER(CBatchItem), _i64]),
     "qfvm_plan_json": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _i32, ctypes.POINTER(CProgram), ctypes.c_char_p, _i64,
                                       ctypes.POINTER(_i64)]),
 }
@@ -358,3 +384,64 @@ class StatePool:
 
 
 pool = StatePool()
+
+
+class PreparedBatch:
+    """The qfvm_batch_item array of a list of (n_qubits, AdjointCall), with output buffers. Built
+    once; rebinding parameters in the calls' programs in place (ParametricProgram.bind) needs no
+    rebuild."""
+
+    def __init__(self, calls):
+        self.calls = list(calls)
+        self.n = len(self.calls)
+        sizes = [c.n_grads for _, c in self.calls]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.grads = np.zeros(max(1, int(self.offsets[-1])), dtype=np.float64)
+        self.expectations = np.zeros(max(1, self.n), dtype=np.float64)
+        self.seconds = np.zeros(max(1, self.n), dtype=np.float64)
+        self.items = (CBatchItem * max(1, self.n))()
+        g0, e0, t0 = self.grads.ctypes.data, self.expectations.ctypes.data, self.seconds.ctypes.data
+        for k, (nq, c) in enumerate(self.calls):
+            ops, gens, n_grads, ch, pos, off, ln, cf, n_terms = c.args
+            it = self.items[k]
+            it.prog = ctypes.pointer(c.cp)
+            it.n_qubits = int(nq)
+            it.grad_ops, it.grad_gen, it.n_grads = ops, gens, n_grads
+            it.chars, it.pos, it.off, it.len, it.coef, it.n_terms = ch, pos, off, ln, cf, n_terms
+            it.grads = g0 + 8 * int(self.offsets[k])
+            it.expectation = e0 + 8 * k
+            it.seconds = t0 + 8 * k
+
+    def grads_of(self, k: int) -> np.ndarray:
+        return self.grads[self.offsets[k]: self.offsets[k + 1]]
+
+
+class NativeBatch:
+    """qfvm_batch: worker threads inside the engine evaluate many circuits' <H> and adjoint
+    gradients per call, with their handles (and plan / graph caches) kept between calls."""
+
+    def __init__(self, dtype: str = "c64", workers: int = 4):
+        h = ctypes.c_void_p()
+        check(lib().qfvm_batch_create(_DTYPES[dtype], int(workers), ctypes.byref(h)))
+        self._h = h
+        self.dtype = "c64" if _DTYPES[dtype] == C64 else "c128"
+        self.workers = int(workers)
+
+    def set_option(self, key: str, value: int, n_qubits: int = 0) -> None:
+        check(lib().qfvm_batch_set_option(self._h, int(n_qubits), key.encode(), int(value)))
+
+    def run(self, prepared: PreparedBatch) -> None:
+        """Evaluates every item; results land in prepared.grads / expectations / seconds."""
+        if prepared.n:
+            check(lib().qfvm_batch_run(self._h, prepared.items, prepared.n))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().qfvm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pylint: disable=broad-except
+            pass

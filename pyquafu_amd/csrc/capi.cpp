@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -1136,6 +1138,9 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->split_rows = value != 0;
         } else if (k == "rows8") {
             s->rows8 = value != 0;
+        } else if (k == "plan_cache") {
+            s->plan_cache_on = value != 0;
+            if (!value) s->plan_cache.reset();
         } else if (k == "stream") {
             s->stream_tiles = value != 0;
         } else if (k == "stagger") {
@@ -1262,6 +1267,171 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
             buf[m] = 0;
         }
         if (reset) s->kstats.clear();
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// Batched evaluation: worker threads, each with its own handles (one per qubit count), run items
+// through the same entry points a caller would use one by one.
+struct qfvm_batch_ctx {
+    int dtype = QFVM_C128;
+    int workers = 1;
+    int device = 0;
+    std::vector<std::vector<std::pair<int, qfvm_handle>>> handles;  // per worker: (n, handle)
+    struct Opt {
+        int n;
+        std::string key;
+        int64_t value;
+    };
+    std::vector<Opt> options;  // in the order they were set
+    int pin_qubits = 20;       // items of at least this many qubits all go to worker 0 (0: none)
+};
+
+namespace {
+
+struct BatchFailure {
+    int code = QFVM_OK;
+    std::string msg;
+};
+
+// the worker's handle for n qubits (created with the batch's options on first use)
+qfvm_handle batch_handle(qfvm_batch_ctx& b, int w, int n, BatchFailure& f) {
+    for (const auto& kv : b.handles[(size_t)w])
+        if (kv.first == n) return kv.second;
+    qfvm_handle h = nullptr;
+    int rc = qfvm_create(n, b.dtype, &h);
+    for (const auto& o : b.options) {
+        if (rc != QFVM_OK) break;
+        if (o.n == 0 || o.n == n) rc = qfvm_set_option(h, o.key.c_str(), o.value);
+    }
+    if (rc != QFVM_OK) {
+        f.code = rc;
+        f.msg = qfvm_last_error();
+        if (h) qfvm_destroy(h);
+        return nullptr;
+    }
+    b.handles[(size_t)w].emplace_back(n, h);
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qfvm_batch_create(int32_t dtype, int32_t workers, qfvm_batch* out) {
+    return guarded([&] {
+        if (!out) throw Error(QFVM_ERR_VALUE, "null output batch");
+        *out = nullptr;
+        if (dtype != QFVM_C64 && dtype != QFVM_C128) throw Error(QFVM_ERR_VALUE, "dtype must be QFVM_C64 or QFVM_C128");
+        if (workers < 1 || workers > 64) throw Error(QFVM_ERR_VALUE, "workers must be in [1, 64]");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error(QFVM_ERR_DEVICE, "no HIP device is visible to this process");
+        auto b = std::make_unique<qfvm_batch_ctx>();
+        QFVM_HIP(hipGetDevice(&b->device));
+        b->dtype = dtype;
+        b->workers = workers;
+        b->handles.resize((size_t)workers);
+        *out = b.release();
+    });
+}
+
+int qfvm_batch_destroy(qfvm_batch b) {
+    return guarded([&] {
+        if (!b) return;
+        for (auto& per : b->handles)
+            for (auto& kv : per) qfvm_destroy(kv.second);
+        delete b;
+    });
+}
+
+int qfvm_batch_set_option(qfvm_batch b, int32_t n_qubits, const char* key, int64_t value) {
+    return guarded([&] {
+        if (!b) throw Error(QFVM_ERR_VALUE, "null batch");
+        if (!key) throw Error(QFVM_ERR_VALUE, "null option key");
+        if (n_qubits < 0 || n_qubits > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
+        if (std::string(key) == "pin_qubits") {  // a scheduling option of the batch itself
+            if (value < 0 || value > 34) throw Error(QFVM_ERR_VALUE, "pin_qubits must be in [0, 34]");
+            b->pin_qubits = (int)value;
+            return;
+        }
+        for (auto& per : b->handles)
+            for (auto& kv : per)
+                if (n_qubits == 0 || kv.first == n_qubits) {
+                    const int rc = qfvm_set_option(kv.second, key, value);
+                    if (rc != QFVM_OK) throw Error(rc, qfvm_last_error());
+                }
+        b->options.push_back({n_qubits, key, value});
+    });
+}
+
+int qfvm_batch_run(qfvm_batch b, const qfvm_batch_item* items, int64_t n_items) {
+    return guarded([&] {
+        if (!b) throw Error(QFVM_ERR_VALUE, "null batch");
+        if (n_items < 0 || (n_items > 0 && !items)) throw Error(QFVM_ERR_VALUE, "bad item array");
+        for (int64_t k = 0; k < n_items; k++) {
+            const qfvm_batch_item& it = items[k];
+            if (!it.prog || !it.expectation || (it.n_grads > 0 && !it.grads))
+                throw Error(QFVM_ERR_VALUE, "batch item without program or outputs");
+            if (it.n_qubits < 0 || it.n_qubits > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
+        }
+        // Circuits large enough to fill the GPU alone gain nothing from running beside each other
+        // and would only stretch each other's latency: they all go to worker 0, largest first.
+        // The rest go largest first (cost ~ 2^n x ops) onto the least-loaded worker.
+        std::vector<int64_t> order((size_t)n_items);
+        std::vector<double> cost((size_t)n_items);
+        for (int64_t k = 0; k < n_items; k++) {
+            order[(size_t)k] = k;
+            cost[(size_t)k] = std::ldexp((double)(items[k].prog->n_ops + 8), items[k].n_qubits);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) { return cost[(size_t)a] > cost[(size_t)c]; });
+        std::vector<std::vector<int64_t>> part((size_t)b->workers);
+        std::vector<double> load((size_t)b->workers, 0.0);
+        const bool pin = b->pin_qubits > 0 && b->workers > 1;
+        for (int64_t k : order) {
+            size_t w = 0;
+            if (!(pin && items[k].n_qubits >= b->pin_qubits))
+                for (size_t j = 1; j < load.size(); j++)
+                    if (load[j] < load[w]) w = j;
+            part[w].push_back(k);
+            load[w] += cost[(size_t)k];
+        }
+        std::vector<BatchFailure> fail((size_t)b->workers);
+        auto work = [&](int w) {
+            BatchFailure& f = fail[(size_t)w];
+            if (hipSetDevice(b->device) != hipSuccess) {
+                f.code = QFVM_ERR_DEVICE;
+                f.msg = "hipSetDevice failed in a batch worker";
+                return;
+            }
+            for (int64_t k : part[(size_t)w]) {
+                const qfvm_batch_item& it = items[k];
+                const auto t0 = std::chrono::steady_clock::now();
+                qfvm_handle h = batch_handle(*b, w, it.n_qubits, f);
+                if (!h) return;
+                int rc = qfvm_init_zero(h);
+                if (rc == QFVM_OK)
+                    rc = qfvm_adjoint(h, it.prog, it.grad_ops, it.grad_gen, it.n_grads, it.chars, it.pos, it.off, it.len,
+                                      it.coef, it.n_terms, it.grads, it.expectation);
+                if (rc != QFVM_OK) {
+                    f.code = rc;
+                    f.msg = qfvm_last_error();
+                    return;
+                }
+                if (it.seconds)
+                    *it.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            }
+        };
+        std::vector<std::thread> threads;
+        for (int w = 1; w < b->workers; w++)
+            if (!part[(size_t)w].empty()) threads.emplace_back(work, w);
+        work(0);
+        for (std::thread& t : threads) t.join();
+        QFVM_HIP(hipSetDevice(b->device));
+        for (const BatchFailure& f : fail)
+            if (f.code != QFVM_OK) throw Error(f.code, f.msg);
     });
 }
 

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "qfvm.h"
@@ -117,6 +118,8 @@ struct Stage {
     unsigned char* cur_host = nullptr;  // host area of the latest stage_host (`host`, or the capture arena)
 };
 
+struct PlanCache;  // plan.cpp
+
 // per-handle state
 struct State {
     int n = 0;
@@ -171,6 +174,9 @@ struct State {
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graph_cache;
+    // fused-pass plans per gate-list structure (option "plan_cache", default on)
+    int plan_cache_on = 1;
+    std::shared_ptr<PlanCache> plan_cache;
     // counters: always on; event timing only when `profiling`
     bool profiling = false;
     std::vector<std::pair<std::string, KStat>> kstats;

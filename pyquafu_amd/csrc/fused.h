@@ -64,6 +64,7 @@ struct FOpHost {
     int32_t slot;       // F_GRAD: output index
     uint64_t pxmask;    // FL_TARGET_PX / F_FLUSH: relabel parity mask of the op's bit g
     int32_t run;        // 1 + index of the diagonal run the op belongs to in its group (0: none)
+    int32_t src;        // index of the gate the op came from in the planned list (-1: none)
 };
 
 // A group is either a compute group (4 tile qubits and their gates) or, in the two-state reverse
@@ -103,9 +104,15 @@ struct PlanItem {
     bool fused;
     FPass pass;
     Gate gate;
+    int src = -1;  // single gates: index of the gate in the planned list
 };
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg);
+// The plan of `gates` from the state's plan cache: a plan is reused for any gate list of the same
+// structure (kinds, qubits, controls, and the gate classes the planner reads: dense / real /
+// diagonal / exchange, zero diagonal entries), with its coefficients re-read from `gates`. Parameter
+// values alone never trigger planning again.
+const std::vector<PlanItem>& cached_plan(State& s, const std::vector<Gate>& gates, const FuseConfig& cfg);
 // Runs a plan: fused passes on the LDS-tiled kernel, single gates on the per-gate kernels. With
 // `lam` set, every gate acts on both s.psi and lam (the reverse sweep) and the plan's gradient
 // markers write grads_dev[slot].

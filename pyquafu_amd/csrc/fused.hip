@@ -387,6 +387,100 @@ __device__ void grad_stage(const V2<R>* phi, const V2<R>* lam, int T, uint64_t g
     }
 }
 
+// Sum over the wavefront of 8 values at once: a transposing butterfly (each exchange halves the
+// values a lane keeps) takes 10 double shuffles instead of 8 x 6. Returns, in every lane, the
+// total of value m = 4 * lane bit 5 + 2 * lane bit 4 + lane bit 3.
+__device__ __forceinline__ double wave_total8(const double (&a)[8], int lane) {
+    const bool b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1;
+    double t4[4], t2[2];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double keep = b5 ? a[k + 4] : a[k], give = b5 ? a[k] : a[k + 4];
+        t4[k] = keep + __shfl_xor(give, 32, 64);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double keep = b4 ? t4[k + 2] : t4[k], give = b4 ? t4[k] : t4[k + 2];
+        t2[k] = keep + __shfl_xor(give, 16, 64);
+    }
+    double t = (b3 ? t2[1] : t2[0]) + __shfl_xor(b3 ? t2[0] : t2[1], 8, 64);
+    t += __shfl_xor(t, 4, 64);
+    t += __shfl_xor(t, 2, 64);
+    t += __shfl_xor(t, 1, 64);
+    return t;
+}
+
+// grad_stage for the split-row kernel, whose threads hold exactly 8 elements each (NT = T / 8):
+// element j of thread tid is j * NT + tid, its global index g_tid | pd.top[j] (no per-element
+// index walk). Im(conj(lam) phi) of the elements is computed once per stage and kept in registers
+// across the markers (Z markers read nothing more), and the wavefront sums of 8 markers take one
+// transposing butterfly. Same partial sums as grad_stage: the same per-thread order over
+// elements, the same exchange partners over the wavefront, the same order over wavefronts.
+template <typename R>
+__device__ void grad_stage8(const V2<R>* phi, const V2<R>* lam, uint64_t g_tid, const PassDesc& pd,
+                            const FOpDev<R>* __restrict__ ops, int cnt, double* red, double* __restrict__ partials,
+                            uint64_t tiles) {
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    double w[8];  // Im(conj(lam) phi) of the thread's elements
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int e = j * NT + tid;
+        const V2<R> p = phi[swz(e)], l = lam[swz(e)];
+        w[j] = (double)l.x * p.y - (double)l.y * p.x;
+    }
+    for (int c0 = 0; c0 < cnt; c0 += 8) {
+        const int cn = min(8, cnt - c0);
+        double acc[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] = 0.0;
+        for (int k = 0; k < cn; k++) {  // one marker at a time keeps few values live
+            const FOpDev<R>& o = ops[c0 + k];
+            // opaque copies: per-element indices and lam reads are redone per marker rather than
+            // hoisted out of the loop into registers the gate groups need
+            int tk = tid;
+            uint64_t gk = g_tid;
+            asm volatile("" : "+v"(tk), "+v"(gk));
+            double a = 0.0;
+            if (o.g == GEN_Z) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const uint64_t gi = gk | pd.top[j];
+                    const double v = ((gi >> o.cm_group) & 1ull) ? -w[j] : w[j];
+                    a += ((gi & o.cm_ext) == o.cm_ext) ? v : 0.0;
+                }
+            } else {
+                const int lb = (int)o.cm_group;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int e = j * NT + tk;
+                    const uint64_t gi = gk | pd.top[j];
+                    const V2<R> l = lam[swz(e)], q = phi[swz(e ^ (1 << lb))];
+                    double v;
+                    if (o.g == GEN_X) {
+                        v = (double)l.x * q.y - (double)l.y * q.x;
+                    } else {  // Y: (Y psi)[e] = +i q where bit lb of e is set, -i q where clear
+                        const double re = (double)l.x * q.x + (double)l.y * q.y;
+                        v = ((e >> lb) & 1) ? re : -re;
+                    }
+                    a += ((gi & o.cm_ext) == o.cm_ext) ? v : 0.0;
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) acc[kk] = kk == k ? a : acc[kk];
+        }
+        const double t = wave_total8(acc, lane);
+        const int m = (((lane >> 5) & 1) << 2) | (((lane >> 4) & 1) << 1) | ((lane >> 3) & 1);
+        if ((lane & 7) == 0 && m < cn) red[m * 8 + wave] = t;
+        __syncthreads();
+        if (tid < cn) {
+            double sum = 0.0;
+            for (int wv = 0; wv < (NT >> 6); wv++) sum += red[tid * 8 + wv];
+            partials[(uint64_t)ops[c0 + tid].flags * tiles + blockIdx.x] = sum;
+        }
+        __syncthreads();
+    }
+}
+
 // Streaming accesses: the state is far larger than L2 and the MALL, and a pass touches each line
 // once, so tile loads and stores carry the non-temporal hint (the split-row kernel's direct
 // HBM -> LDS loads set it in their cache-policy operand).
@@ -954,9 +1048,9 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
     for (int gi = 0; gi < pd.ngroups; gi++) {
         if constexpr (NS == 2) {
             if ((pd.stage_mask >> gi) & 1u) {
-                grad_stage<R>(tile, tile + T, T, base | local_to_global(tid, pd), pd, ops + pd.gop[gi],
-                                   pd.gop[gi + 1] - pd.gop[gi], red, partials, tiles);
-                continue;  // grad_stage ends with a barrier
+                grad_stage8<R>(tile, tile + T, base | local_to_global(tid, pd), pd, ops + pd.gop[gi],
+                               pd.gop[gi + 1] - pd.gop[gi], red, partials, tiles);
+                continue;  // grad_stage8 ends with a barrier
             }
         }
         const int s0 = pd.gbits[gi][0], s1 = pd.gbits[gi][1], s2 = pd.gbits[gi][2], s3 = pd.gbits[gi][3];

@@ -153,7 +153,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
         while (first < gates.size() && done[first]) first++;
         if (first >= gates.size()) break;
         if (info[first].multi) {
-            plan.push_back(PlanItem{false, FPass{}, gates[first]});
+            plan.push_back(PlanItem{false, FPass{}, gates[first], (int)first});
             done[first] = 1;
             continue;
         }
@@ -228,6 +228,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                 for (int j = 0; j < FUSE_GROUP_BITS; j++) fg.S[j] = -1;
                 for (int gi : G.gates) {
                     FOpHost o{};
+                    o.src = -1;
                     o.kind = F_GRAD;
                     o.g = gates[gi].gen;
                     o.cm_ext = gates[gi].cmask;
@@ -253,9 +254,11 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             auto pending = [&](int j) { return M[j] != 0 || ((C >> j) & 1); };
             for (int gi : G.gates) {
                 FOpHost o = convert(gates[gi], info[gi], fg.S);
+                o.src = gi;
                 for (int j = 0; j < FUSE_GROUP_BITS; j++) {
                     if (((o.cm_group >> j) & 1) && pending(j)) {
                         FOpHost f{};
+                        f.src = -1;
                         f.kind = F_FLUSH;
                         f.g = j;
                         f.pxmask = M[j];
@@ -293,6 +296,105 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
     return plan;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Plan cache. The planner reads, per gate, its kind, qubits, controls and class (info_of); the
+// coefficients only reach the plan's ops through convert. A plan is therefore valid for every gate
+// list with the same per-gate structure words below, once each op's coefficients are re-read from
+// its source gate.
+struct PlanCache {
+    struct Entry {
+        uint64_t hash;
+        std::vector<uint64_t> sig;
+        std::vector<PlanItem> plan;
+        uint64_t last;
+    };
+    std::vector<Entry> entries;
+    std::vector<uint64_t> sig;  // scratch
+    uint64_t clock = 0;
+};
+constexpr size_t PLAN_CACHE_MAX = 64;  // plans kept per handle (least recently used dropped first)
+
+namespace {
+
+void plan_signature(const std::vector<Gate>& gates, int n, const FuseConfig& cfg, std::vector<uint64_t>& sig) {
+    sig.clear();
+    sig.reserve(gates.size() * 5 + 2);
+    sig.push_back((uint64_t)n | ((uint64_t)cfg.tile_bits << 8) | ((uint64_t)cfg.low_bits << 16) |
+                  ((uint64_t)cfg.max_groups << 24) | ((uint64_t)cfg.group_bits << 32));
+    sig.push_back((uint64_t)(uint32_t)cfg.max_cost);
+    for (const Gate& g : gates) {
+        const GInfo gi = info_of(g);
+        const uint64_t cls = (gi.diag ? 1u : 0u) | (gi.multi ? 2u : 0u) | (gi.grad ? 4u : 0u) |
+                             (gi.diag && gi.d0 == cd(0.0, 0.0) ? 8u : 0u) | (gi.diag && gi.d1 == cd(0.0, 0.0) ? 16u : 0u);
+        sig.push_back((uint64_t)(uint32_t)g.kind | ((uint64_t)(uint32_t)g.target << 8) | ((uint64_t)(uint32_t)g.gen << 16) |
+                      (cls << 24) | ((uint64_t)(uint32_t)g.slot << 32));
+        sig.push_back(g.cmask);
+        sig.push_back(gi.N);
+        sig.push_back(gi.D);
+        if (gi.multi) {
+            sig.push_back(g.targets.size() | ((uint64_t)g.ctrls.size() << 32));
+            for (int t : g.targets) sig.push_back((uint64_t)(uint32_t)t);
+        }
+    }
+}
+
+// re-reads every op's coefficients (and every single gate) from `gates`
+void refresh_plan(std::vector<PlanItem>& plan, const std::vector<Gate>& gates) {
+    for (PlanItem& it : plan) {
+        if (!it.fused) {
+            it.gate = gates[(size_t)it.src];
+            continue;
+        }
+        for (FGroup& fg : it.pass.groups) {
+            if (fg.stage) continue;
+            for (FOpHost& o : fg.ops) {
+                if (o.src < 0) continue;
+                switch (o.kind) {
+                    case F_DENSE:
+                    case F_REAL:
+                    case F_DIAG:
+                    case F_PHASE:
+                    case F_SWAP: {
+                        const Gate& g = gates[(size_t)o.src];
+                        const FOpHost c = convert(g, info_of(g), fg.S);
+                        for (int k = 0; k < 4; k++) o.m[k] = c.m[k];
+                        break;
+                    }
+                    default: break;  // relabelings, flushes: no coefficients
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+const std::vector<PlanItem>& cached_plan(State& s, const std::vector<Gate>& gates, const FuseConfig& cfg) {
+    if (!s.plan_cache) s.plan_cache = std::make_shared<PlanCache>();
+    PlanCache& pc = *s.plan_cache;
+    plan_signature(gates, s.n, cfg, pc.sig);
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t w : pc.sig) h = (h ^ w) * 1099511628211ull;
+    pc.clock++;
+    for (PlanCache::Entry& e : pc.entries) {
+        if (e.hash == h && e.sig == pc.sig) {
+            e.last = pc.clock;
+            refresh_plan(e.plan, gates);
+            s.kstats[(size_t)s.stat_index("plan_cache_hit")].second.launches++;
+            return e.plan;
+        }
+    }
+    if (pc.entries.size() >= PLAN_CACHE_MAX) {
+        size_t old = 0;
+        for (size_t k = 1; k < pc.entries.size(); k++)
+            if (pc.entries[k].last < pc.entries[old].last) old = k;
+        pc.entries.erase(pc.entries.begin() + (std::ptrdiff_t)old);
+    }
+    pc.entries.push_back(PlanCache::Entry{h, pc.sig, plan_gates(gates, s.n, cfg), pc.clock});
+    s.kstats[(size_t)s.stat_index("plan_cache_miss")].second.launches++;
+    return pc.entries.back().plan;
+}
+
 static FuseConfig config_of(const State& s, int states) {
     FuseConfig cfg;
     // the two-state sweep holds both tiles in LDS, so an explicit tile size is halved for it
@@ -313,7 +415,11 @@ void apply_gates(State& s, const std::vector<Gate>& gates) {
         for (const Gate& g : gates) launch_gate(s, g);
         return;
     }
-    execute_plan(s, plan_gates(gates, s.n, config_of(s, 1)));
+    const FuseConfig cfg = config_of(s, 1);
+    if (s.plan_cache_on)
+        execute_plan(s, cached_plan(s, gates, cfg));
+    else
+        execute_plan(s, plan_gates(gates, s.n, cfg));
 }
 
 void apply_gates2(State& s, const std::vector<Gate>& gates, void* lam, double* grads_dev) {
@@ -330,7 +436,11 @@ void apply_gates2(State& s, const std::vector<Gate>& gates, void* lam, double* g
         }
         return;
     }
-    execute_plan(s, plan_gates(gates, s.n, config_of(s, 2)), lam, grads_dev);
+    const FuseConfig cfg = config_of(s, 2);
+    if (s.plan_cache_on)
+        execute_plan(s, cached_plan(s, gates, cfg), lam, grads_dev);
+    else
+        execute_plan(s, plan_gates(gates, s.n, cfg), lam, grads_dev);
 }
 
 // ------------------------------------------------------------------------------------------------

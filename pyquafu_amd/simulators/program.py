@@ -124,6 +124,16 @@ class ProgramBuilder:
         return Program(ops, idx, mat, **kw)
 
 
+def gate_matrix(op) -> np.ndarray:
+    """The matrix a GENERAL / MULTI op carries: the target matrix (reverse_order=True) of a
+    controlled gate, else `matrix`."""
+    if hasattr(op, "_targ_matrix"):
+        m = op._get_targ_matrix(reverse_order=True)  # pylint: disable=protected-access
+    else:
+        m = op.matrix
+    return np.asarray(m, dtype=np.complex128)
+
+
 def _emit_pyop(b: ProgramBuilder, op) -> str:
     """from_pyops (instructions.hpp:229-299) for one non-wrapper instruction; returns its kind."""
     name = op.name.lower()
@@ -165,11 +175,7 @@ def _emit_pyop(b: ProgramBuilder, op) -> str:
     if code is not None:
         b.add(code, qubits=pos, n_ctrl=n_ctrl, param=float(paras[0]) if paras else 0.0)
         return "gate"
-    if hasattr(op, "_targ_matrix"):
-        m = op._get_targ_matrix(reverse_order=True)  # pylint: disable=protected-access
-    else:
-        m = op.matrix
-    m = np.asarray(m, dtype=np.complex128)
+    m = gate_matrix(op)
     n_targ = len(pos) - n_ctrl
     if n_targ < 1:
         raise RuntimeError("Invalid target number")

@@ -1,0 +1,153 @@
+"""Plan cache, parameter rebinding and the batch entry (qfvm_batch_run) on the GPU.
+
+The plan cache reuses a fused-pass plan for a gate list of the same structure and re-reads the
+coefficients: results after a rebinding must equal the oracle's for the new values, and a change
+of gate class (a dense matrix becoming diagonal) must plan again. The batch entry evaluates many
+circuits on engine worker threads; its results must equal single qfvm_adjoint calls and the
+oracle, before and after the parameters are rebound in place."""
+
+import numpy as np
+import pytest
+
+from oracle import gradients as og
+from oracle.cpu import CpuState
+from pyquafu_amd import _native, workloads
+from pyquafu_amd.batch import Autotuner, BatchRunner, compile_item
+from pyquafu_amd.simulators.binding import ParametricProgram
+from pyquafu_amd.simulators.program import GENERAL, ProgramBuilder
+from tests.conftest import rand_state
+
+pytestmark = pytest.mark.gpu
+
+
+def cache_counts(dev):
+    p = dev.profile(reset=False)
+    return p.get("plan_cache_hit", {}).get("launches", 0), p.get("plan_cache_miss", {}).get("launches", 0)
+
+
+@pytest.mark.parametrize("dtype,n", [("c128", 16), ("c64", 17), ("c128", 13)])
+def test_plan_cache_rebinding_matches_oracle(dtype, n):
+    qc, _ = workloads.hea_ansatz(n, 4, parametrised=True)
+    pp = ParametricProgram(qc)
+    rng = np.random.Generator(np.random.PCG64(4000 + n))
+    psi0 = rand_state(n, rng)
+    dev = _native.DeviceState(n, dtype)
+    tol = 1e-12 if dtype == "c128" else 2e-5
+    try:
+        for step in range(3):
+            pp.bind(rng.uniform(-np.pi, np.pi, size=pp.n_vars))
+            cpu = CpuState(n)
+            cpu.load(psi0)
+            cpu.apply(pp.prog)
+            dev.load(psi0)
+            dev.apply(pp.prog)
+            err = float(np.abs(dev.store() - cpu.store()).max())
+            hit, miss = cache_counts(dev)
+            print(f"[plan cache {dtype} n={n}] step {step} err {err:.1e} hits {hit} misses {miss}", flush=True)
+            assert err < tol
+            assert miss == 1 and hit == step
+    finally:
+        dev.close()
+
+
+def test_plan_cache_replans_on_class_change():
+    """A GENERAL matrix that turns diagonal changes the plan's structure (a diagonal takes no tile
+    slot): the cache must miss, and both results must match the oracle."""
+    n = 14
+    rng = np.random.Generator(np.random.PCG64(4100))
+
+    def prog(diagonal):
+        b = ProgramBuilder()
+        for q in range(n):
+            th = rng.uniform(-np.pi, np.pi)
+            m = np.diag([1.0, np.exp(1j * th)]) if diagonal and q % 3 == 0 else np.array(
+                [[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]], dtype=complex) * np.exp(0.3j)
+            b.add(GENERAL, [q], matrix=m)
+        for q in range(n - 1):
+            b.add(GENERAL, [q, q + 1], n_ctrl=1, matrix=np.array([[0, 1], [1, 0]], dtype=complex))
+        return b.program(n_qubits=n)
+
+    psi0 = rand_state(n, rng)
+    dev = _native.DeviceState(n, "c128")
+    try:
+        for k, p in enumerate([prog(False), prog(False), prog(True)]):
+            cpu = CpuState(n)
+            cpu.load(psi0)
+            cpu.apply(p)
+            dev.load(psi0)
+            dev.apply(p)
+            assert np.abs(dev.store() - cpu.store()).max() < 1e-12
+        hit, miss = cache_counts(dev)
+        assert (hit, miss) == (1, 2)
+        dev.set_option("plan_cache", 0)  # off: plans every call, same results
+        dev.load(psi0)
+        dev.apply(p)
+        assert np.abs(dev.store() - cpu.store()).max() < 1e-12
+        assert cache_counts(dev) == (1, 2)
+    finally:
+        dev.close()
+
+
+def test_adjoint_rebinding_hits_cache_and_matches_oracle():
+    n = 12
+    qc, _ = workloads.hea_ansatz(n, 3, parametrised=True)
+    item = compile_item(qc, workloads.tfim_hamiltonian(n))
+    rng = np.random.Generator(np.random.PCG64(4200))
+    dev = _native.DeviceState(n, "c128")
+    try:
+        for step in range(3):
+            item.bind(rng.uniform(-np.pi, np.pi, size=item.pp.n_vars))
+            dev.init_zero()
+            g, e = dev.adjoint_prepared(item.call)  # the call's arguments see the rebound program
+            want, e_want = og.adjoint(item.prog, n, item.ops, item.gens, item.terms, item.coeffs)
+            assert abs(e - e_want) < 1e-11
+            assert np.abs(g - want).max() < 1e-10
+        hit, miss = cache_counts(dev)
+        assert miss == 2 and hit == 4  # forward and reverse plan, planned once each
+    finally:
+        dev.close()
+
+
+@pytest.mark.parametrize("dtype,graphs", [("c128", False), ("c128", True), ("c64", True)])
+def test_batch_entry_matches_oracle_across_rebinding(dtype, graphs):
+    circuits = workloads.batch_circuits(count=20, seed=4300, n_range=(6, 13), depth_range=(2, 5))
+    items = [compile_item(qc, workloads.z_sum(qc.num)) for qc, _ in circuits]
+    runner = BatchRunner(dtype=dtype, workers=3, tuner=Autotuner(probe_layers=2), graphs=graphs)
+    rng = np.random.Generator(np.random.PCG64(4301))
+    try:
+        for step in range(3):
+            values = [rng.uniform(-np.pi, np.pi, size=it.pp.n_vars) for it in items] if step else None
+            results, wall = runner.run(items, values)
+            assert wall > 0
+            worst = 0.0
+            for it, r in zip(items, results):
+                want, e_want = og.adjoint(it.prog, it.n, it.ops, it.gens, it.terms, it.coeffs)
+                tol = 1e-10 if dtype == "c128" else 1e-5 * it.n
+                worst = max(worst, abs(r.expectation - e_want) / tol, float(np.abs(r.grads - want).max()) / tol)
+                assert abs(r.expectation - e_want) < tol and np.abs(r.grads - want).max() < tol
+                assert r.latency_s > 0
+            print(f"[batch {dtype} graphs={graphs}] step {step} wall {wall * 1e3:.2f} ms worst/tol {worst:.3f}",
+                  flush=True)
+    finally:
+        runner.close()
+
+
+def test_batch_entry_reports_errors():
+    """A bad item (op index out of range) fails the whole call with the engine's message; the
+    context stays usable."""
+    circuits = workloads.batch_circuits(count=4, seed=4400, n_range=(6, 8), depth_range=(2, 2))
+    items = [compile_item(qc, workloads.z_sum(qc.num)) for qc, _ in circuits]
+    nb = _native.NativeBatch("c128", 2)
+    try:
+        bad = _native.AdjointCall(items[1].prog, [10 ** 6], "Y", items[1].terms, items[1].coeffs)
+        pb = _native.PreparedBatch([(it.n, it.call) for it in items[:1]] + [(items[1].n, bad)])
+        with pytest.raises(_native.EngineError, match="out of range"):
+            nb.run(pb)
+        good = _native.PreparedBatch([(it.n, it.call) for it in items])
+        nb.run(good)
+        for k, it in enumerate(items):
+            want, e_want = og.adjoint(it.prog, it.n, it.ops, it.gens, it.terms, it.coeffs)
+            assert abs(good.expectations[k] - e_want) < 1e-11
+            assert np.abs(good.grads_of(k) - want).max() < 1e-10
+    finally:
+        nb.close()

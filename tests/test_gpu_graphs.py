@@ -159,7 +159,7 @@ def test_batch_runner_graph_steps_match_eager():
     items = []
     for qc, _ in circuits:
         items.append(batch.compile_item(qc, Hamiltonian([PauliOp(f"Z{i}") for i in range(qc.num)])))
-    eager = batch.BatchRunner(dtype="c64", workers=3)
+    eager = batch.BatchRunner(dtype="c64", workers=3, graphs=False)
     graph = batch.BatchRunner(dtype="c64", workers=3, graphs=True)
     try:
         want, _ = eager.run(items)

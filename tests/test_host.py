@@ -220,24 +220,36 @@ def test_compile_ops_for_applyop():
     assert prog.ops["code"].tolist() == [P.RX, P.CZ]
 
 
-def test_batch_partition_is_deterministic_and_balanced():
-    """Graph-mode batches give each worker slot a fixed share (longest first onto the least-loaded
-    slot), so a circuit evaluated again lands on the handle that holds its graph."""
-    from types import SimpleNamespace
+def test_parametric_program_bind_equals_recompile():
+    """Rebinding writes the same op stream and matrix pool as compiling the circuit again with the
+    new values: named rotations of one variable through the vectorised slot map, expression angles
+    and matrix gates (U3, controlled rotations) re-derived from their gates."""
+    from pyquafu_amd.circuits import QuantumCircuit
+    from pyquafu_amd.elements import CRZGate, CZGate, RXGate, RYGate, U3Gate
+    from pyquafu_amd.elements.parameters import Parameter
+    from pyquafu_amd.simulators.binding import ParametricProgram
+    from pyquafu_amd.simulators.program import compile_gates
 
-    from pyquafu_amd.batch import BatchRunner
+    a, b, c = Parameter("a", 0.1), Parameter("b", 0.2), Parameter("c", -0.3)
+    qc = QuantumCircuit(4)
+    qc << U3Gate(0, a, b, 0.3)
+    qc << CRZGate(0, 1, a * 2)
+    qc << RYGate(2, b)
+    qc << RXGate(3, c)
+    qc << CZGate(1, 2)
+    qc << RYGate(1, c)
+    pp = ParametricProgram(qc)
+    assert [v.name for v in pp.variables] == ["a", "b", "c"]
+    assert len(pp.fast_rows) == 3 and len(pp.slow) == 2
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        vals = rng.uniform(-np.pi, np.pi, size=3)
+        pp.bind(vals)
+        want, _ = compile_gates(qc.gates, 4)
+        assert np.array_equal(pp.prog.ops, want.ops)
+        assert np.array_equal(pp.prog.mat, want.mat)
+        assert [v.value for v in (a, b, c)] == list(vals)
+    with pytest.raises(ValueError):
+        pp.bind([0.0, 1.0])
 
-    rng = np.random.default_rng(5)
-    items = [SimpleNamespace(n=int(rng.integers(10, 23)), call=SimpleNamespace(cp=SimpleNamespace(n_ops=int(rng.integers(20, 400)))))
-             for _ in range(97)]
-    runner = BatchRunner(dtype="c64", workers=4, graphs=True)
-    try:
-        parts = runner._partition(items)  # pylint: disable=protected-access
-        assert parts == runner._partition(items)  # pylint: disable=protected-access
-        assert sorted(k for p in parts for k in p) == list(range(len(items)))
-        cost = [(1 << it.n) * (it.call.cp.n_ops + 8) for it in items]
-        loads = [sum(cost[k] for k in p) for p in parts]
-        # LPT: no slot exceeds the mean load by more than the largest single item
-        assert max(loads) <= sum(loads) / len(loads) + max(cost)
-    finally:
-        runner.close()
+
