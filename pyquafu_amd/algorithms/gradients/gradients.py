@@ -68,6 +68,32 @@ def adjoint_program(qc):
     return prog, [rows[i] for i in diff], "".join(generator_of(gates[i]) for i in diff), diff
 
 
+def _gate_stamp(gates):
+    """Identity of a gate list for the compiled-program cache: the gate objects and their numeric
+    parameters (symbolic ones are rebound on every call)."""
+    return tuple((id(g), tuple(p if isinstance(p, (int, float)) else id(p) for p in g.paras)) for g in gates)
+
+
+def bound_adjoint_program(qc):
+    """adjoint_program with the circuit compiled once: the ParametricProgram is kept on the circuit
+    while its gate list (gate objects, numeric parameters) stays the same, and each call only
+    rebinds the current variable values into it (training loops re-running one circuit with new
+    parameters). Returns (program, op rows, generators, gate indices)."""
+    from ...simulators.binding import ParametricProgram  # pylint: disable=import-outside-toplevel
+
+    stamp = _gate_stamp(qc.gates)
+    cached = getattr(qc, "_pq_bound", None)
+    if cached is None or cached[0] != stamp:
+        pp = ParametricProgram(qc)
+        diff = [i for i, op in enumerate(pp.gates) if _is_symbolic(op)]
+        cached = (stamp, pp, [pp.rows[i] for i in diff], "".join(generator_of(pp.gates[i]) for i in diff), diff)
+        qc._pq_bound = cached  # pylint: disable=protected-access
+    else:
+        pp = cached[1]
+        pp.bind([v.value for v in pp.variables])
+    return cached[1].prog, cached[2], cached[3], cached[4]
+
+
 def grad_adjoint(qc, hamiltonian, psi_in=np.array([], dtype=complex), *, precision: str = "c128",
                  return_expectation: bool = False, checkpoints: int = 0):
     """Reverse-mode gradient (arXiv:2009.02823) of <H> with respect to qc.variables' order of
@@ -77,7 +103,7 @@ def grad_adjoint(qc, hamiltonian, psi_in=np.array([], dtype=complex), *, precisi
     if not qc._parameter_grads:  # pylint: disable=protected-access
         qc.get_parameter_grads()
     para_grads = qc._calc_parameter_grads()  # pylint: disable=protected-access
-    prog, ops, gens, diff = adjoint_program(qc)
+    prog, ops, gens, diff = bound_adjoint_program(qc)
     terms, coeffs = hamiltonian_terms(hamiltonian)
     dev = _call(_native.pool.get, qc.num, precision)
     _call(dev.set_option, "checkpoints", int(checkpoints))

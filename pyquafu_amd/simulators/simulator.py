@@ -148,6 +148,32 @@ class NoiseSVSimulator(Simulator):
         new_qc << temp_qc.wrap()  # pylint: disable=pointless-statement
         return new_qc
 
+    def run_once(self, qc, psi, hamiltonian=None, rng=None):
+        """One trajectory from `psi` (an empty array: |0...0>): returns (the sampled bitstring of
+        qc.measures, the coefficient-weighted Pauli expectations or None), as the reference's
+        run_once (simulator.py:152-164). `rng` (an extension) supplies the uniforms."""
+        prog = self.trajectory_program(qc)
+        n = qc.num
+        has_input = psi is not None and np.asarray(psi).size != 0
+        if has_input and np.asarray(psi).size != (1 << n):
+            raise ValueError(f"input state length {np.asarray(psi).size} does not match {n} qubits")
+        rng = np.random.default_rng(self.seed) if rng is None else rng
+        dev = _call(_native.pool.get, n, self.precision)
+        if has_input:
+            _call(dev.load, np.asarray(psi, dtype=np.complex128))
+        else:
+            _call(dev.init_zero)
+        u = rng.random(prog.n_random + 1)
+        _call(dev.run, prog, u[:-1], 0, True)
+        sample = _call(dev.sample, u[-1:])
+        meas = [(int(q), int(c)) for q, c in qc.measures.items()]
+        key = next(iter(bitstring_counts(np.asarray(sample, dtype=np.uint64), meas)))
+        if not hamiltonian:
+            return key, None
+        paulis = hamiltonian.paulis
+        res = _call(dev.expect, [(p.paulistr, [int(x) for x in p.pos]) for p in paulis])
+        return key, [float(v) * p.coeff for v, p in zip(res, paulis)]
+
     def run(self, qc, psi: np.ndarray = np.array([]), shots: int = 0, hamiltonian=None):  # pylint: disable=arguments-differ
         prog = self.trajectory_program(qc)
         n = qc.num

@@ -339,3 +339,36 @@ def test_adjoint_checkpoints_more_than_ops():
     got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, options=[("checkpoints", 64)])
     assert abs(e - e_want) < 1e-11
     assert np.abs(got - want).max() < 1e-10
+
+
+def test_qnn_module_wrapper_forward_backward():
+    """QuantumNeuralNetwork + ModuleWrapper (interface/torch.py:92-130): inputs concatenated with
+    the trainable weights form each row's parameters; outputs <Z_i> equal the oracle's and the
+    weight gradient equals the sum over rows of the input-row gradients' weight part."""
+    import torch
+
+    from oracle import dense
+    from pyquafu_amd.algorithms.interface.torch import ModuleWrapper, QuantumNeuralNetwork
+    from pyquafu_amd.elements import CZGate, RXGate, RYGate
+    from pyquafu_amd.elements.parameters import Parameter
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    n = 3
+    layers = [RXGate(q, Parameter(f"x{q}", 0.0, tunable=False)) for q in range(n)]
+    layers += [RYGate(q, Parameter(f"w{q}", 0.1 * q)) for q in range(n)] + [CZGate(0, 1), CZGate(1, 2)]
+    qnn = QuantumNeuralNetwork(n, layers)
+    assert qnn.num_tunable_parameters == n and tuple(qnn.weights.shape) == (1, n)
+    model = ModuleWrapper(qnn)
+    rng = np.random.Generator(np.random.PCG64(1800))
+    x = torch.tensor(rng.uniform(-1, 1, size=(2, n)), dtype=torch.float64)
+    out = model(x)
+    assert tuple(out.shape) == (2, n)
+    w = model.weights.detach().numpy()[0]
+    for r in range(2):
+        qnn.circ._update_params(list(x[r].numpy()) + list(w))  # pylint: disable=protected-access
+        psi, _ = dense.run_dense(compile_circuit(qnn.circ), n)
+        want = [og.expectation(psi, n, [("Z", [i])], [1.0]) for i in range(n)]
+        assert np.abs(out[r].detach().numpy() - want).max() < 1e-12
+    out.sum().backward()
+    assert model.weights.grad is not None and tuple(model.weights.grad.shape) == (1, n)
+    assert np.isfinite(model.weights.grad.numpy()).all()

@@ -204,3 +204,20 @@ def test_reference_noise_circuit_ghz():
     assert sum(counts.values()) == 3000 and set(counts) <= {"00", "01", "10", "11"}
     assert counts.get("00", 0) + counts.get("11", 0) > 0.9 * 3000
     assert abs(counts.get("00", 0) - counts.get("11", 0)) < 0.1 * 3000
+
+
+def test_noise_run_once_matches_run():
+    """run_once (simulator.py:152-164): one trajectory's sample and weighted expectations; with the
+    same uniforms it equals a one-shot run."""
+    rng = np.random.Generator(np.random.PCG64(91))
+    n = 4
+    qc = noisy_circuit(n, 20, rng)
+    qc.measure([0, 2], [0, 1])
+    ham = Hamiltonian([PauliOp("Z0", 0.5), PauliOp("X1 Y3", -1.0)])
+    sim = NoiseSVSimulator(seed=21)
+    key, res = sim.run_once(qc, np.array([]), ham, rng=np.random.default_rng(21))
+    one = NoiseSVSimulator(seed=21).run(qc, shots=1, hamiltonian=ham)
+    assert one["counts"] == {key: 1}
+    assert np.abs(np.array(res) - np.array(one["pauli_expects"])).max() < 1e-12
+    key2, none = sim.run_once(qc, np.array([]), None, rng=np.random.default_rng(21))
+    assert key2 == key and none is None

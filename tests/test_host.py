@@ -253,3 +253,24 @@ def test_parametric_program_bind_equals_recompile():
         pp.bind([0.0, 1.0])
 
 
+
+
+def test_bound_adjoint_program_rebinds_and_recompiles():
+    """grad_adjoint's compiled-program cache: the same gate list with new variable values rebinds
+    the cached program (same arrays, new values); a changed gate list compiles afresh."""
+    from pyquafu_amd import workloads
+    from pyquafu_amd.algorithms.gradients.gradients import adjoint_program, bound_adjoint_program
+    from pyquafu_amd.elements import RYGate
+    from pyquafu_amd.elements.parameters import Parameter
+
+    qc, _ = workloads.hea_ansatz(6, 2, parametrised=True)
+    p1, ops1, gens1, diff1 = bound_adjoint_program(qc)
+    qc._update_params(list(np.linspace(-1, 1, len(qc.variables))))  # pylint: disable=protected-access
+    p2, ops2, gens2, diff2 = bound_adjoint_program(qc)
+    assert p2 is p1 and (ops2, gens2, diff2) == (ops1, gens1, diff1)
+    want = adjoint_program(qc)
+    assert np.array_equal(p2.ops, want[0].ops) and (ops2, gens2, diff2) == tuple(want[1:])
+    qc << RYGate(0, Parameter("extra", 0.25))
+    p3, ops3, _, _ = bound_adjoint_program(qc)
+    assert p3 is not p1 and len(ops3) == len(ops1) + 1
+    assert np.array_equal(p3.ops, adjoint_program(qc)[0].ops)
