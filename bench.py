@@ -32,10 +32,6 @@ import subprocess
 import sys
 import time
 
-# The batch leg runs circuits on 8 engine streams; HIP maps a process's streams onto
-# GPU_MAX_HW_QUEUES hardware queues (4 by default), and kernels that share a queue run one after
-# another. Read once at HIP initialisation, so it is set before anything touches the GPU.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 import numpy as np
 
@@ -434,6 +430,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-circuit leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (1..32; 0 leaves the environment alone): HIP maps "
+                         "the batch leg's engine streams onto that many hardware queues, and kernels that share "
+                         "a queue run one after another")
     ap.add_argument("--batch-pin", type=int, default=21,
                     help="circuits of at least this many qubits run one after another on one worker (0: off)")
     ap.add_argument("--no-graph-batch", action="store_true",
@@ -445,6 +445,11 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the statevector "
               "engine holds the whole state on one MI355X; no multi-GPU figure is produced)", file=sys.stderr)
         return 2
+    if args.hw_queues:  # read once at HIP initialisation, which the first engine call below performs
+        if not 1 <= args.hw_queues <= 32:
+            print("--hw-queues must be in 1..32", file=sys.stderr)
+            return 2
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.copy_calibration:
         return copy_calibration()
 
