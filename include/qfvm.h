@@ -318,6 +318,8 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 coefficients; 0 = plan every call (and drop the kept plans)
  *   "rows8"       1 = (default) complex128 one-state passes of 2^10..2^13-amplitude tiles hold one
  *                 8-amplitude row (3-qubit groups) per lane instead of split rows, 0 = split rows
+ *   "rows8_c64"   1 = complex64 one-state passes of such tiles do the same (with "rows8" on),
+ *                 0 = (default) 16-amplitude rows
  *   "stream"      1 = persistent fused-pass kernel that prefetches the next tile (complex128,
  *                 one-state passes of 2^11..2^13-amplitude tiles, when split_rows does not apply),
  *                 0 = one workgroup per tile

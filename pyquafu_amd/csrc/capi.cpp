@@ -590,7 +590,7 @@ static uint64_t adjoint_key(const State& s, const qfvm_program* prog, const int6
                             const int32_t* len, int64_t n_terms) {
     uint64_t h = 1469598103934665603ull;
     const int32_t opts[] = {s.n, s.dtype, s.mfma, s.fusion, s.tile_bits, s.low_bits, s.max_groups, s.pass_cost, s.split_rows,
-                            s.stream_tiles, s.stagger, s.stagger_shift, s.rows8};
+                            s.stream_tiles, s.stagger, s.stagger_shift, s.rows8, s.rows8_c64};
     h = fnv(h, opts, sizeof(opts));
     h = fnv(h, &prog->n_ops, sizeof(prog->n_ops));
     for (int64_t k = 0; k < prog->n_ops; k++) {
@@ -1138,6 +1138,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->split_rows = value != 0;
         } else if (k == "rows8") {
             s->rows8 = value != 0;
+        } else if (k == "rows8_c64") {
+            s->rows8_c64 = value != 0;
         } else if (k == "plan_cache") {
             s->plan_cache_on = value != 0;
             if (!value) s->plan_cache.reset();

@@ -42,7 +42,7 @@ struct PassDesc {
     uint32_t gpxc[FUSE_MAX_GROUPS];                   // relabel constants at each group's end
     uint64_t top[8];     // global offsets of local index j << (b - 3): the tile's 3 highest local bits
     uint64_t top16[16];  // the same for j << (b - 4): the 4 highest
-    // row-of-8 passes: byte offsets swz(member m's group bits) * 16 of each group, the uniform
+    // row-of-8 passes: byte offsets swz(member m's group bits) x amplitude bytes, the uniform
     // relabel offset of groups without lane parities, and per-group flags (bit gi): the group needs
     // the lane's global index (lane-dependent ops or relabel parities) / its relabeling is per lane
     int32_t goff[FUSE_MAX_GROUPS][8];
@@ -1092,8 +1092,8 @@ __global__ void __launch_bounds__(NS == 1 ? 1024 : 512) k_fused_half(V2<R>* __re
 }
 
 // ------------------------------------------------------------------------------------------------
-// Row-of-8 pass kernel (complex128 one-state passes of 2^10 .. 2^13-amplitude tiles, 3-qubit
-// groups). One lane holds the 8 amplitudes that differ only in its group's 3 bits, so no gate
+// Row-of-8 pass kernel (one-state passes of 2^10 .. 2^13-amplitude tiles, 3-qubit groups;
+// complex128 by default, complex64 with option "rows8_c64"). One lane holds the 8 amplitudes that differ only in its group's 3 bits, so no gate
 // needs another lane's data (the split-row kernel pairs lanes for its 4th group bit, and each of
 // those gates pays 32 DPP moves); a 2^b tile still gets 2^(b-3) threads, and the load / store
 // skeleton is the split-row kernel's.
@@ -1372,18 +1372,25 @@ __device__ __forceinline__ uint64_t local_to_global_bits(int l, const PassDesc& 
     return g;
 }
 
-// Direct HBM -> LDS loads of the tile at global offset `base` (as k_fused_half): slot lh + tid
-// receives element lh + swz(tid), the 8 loads cover the tile's 3 highest local bits. lo_sw is the
-// thread's offset local_to_global_low(swz(tid)).
+// Loads the tile at global offset `base` into LDS, element e at slot swz(e) (as k_fused_half):
+// complex128 with direct HBM -> LDS loads (slot lh + tid receives element lh + swz(tid)), complex64
+// through registers (8-byte loads); the 8 loads cover the tile's 3 highest local bits.
 template <typename R>
-__device__ __forceinline__ void pass8_load(const V2<R>* __restrict__ psi, V2<R>* tile, uint64_t base, uint64_t lo_sw,
+__device__ __forceinline__ void pass8_load(const V2<R>* __restrict__ psi, V2<R>* tile, uint64_t base, int tid,
                                            const PassDesc& pd, int NT, int wave) {
     __builtin_amdgcn_s_setprio(2);
+    if constexpr (sizeof(R) == 8) {
+        const uint64_t g = base | local_to_global_low(swz(tid), pd);
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const void* src = (const void*)(psi + (base | lo_sw | pd.top[j]));
-        auto* dst = (__attribute__((address_space(3))) void*)(tile + j * NT + wave * 64);
-        __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
+        for (int j = 0; j < 8; j++) {
+            const void* src = (const void*)(psi + (g | pd.top[j]));
+            auto* dst = (__attribute__((address_space(3))) void*)(tile + j * NT + wave * 64);
+            __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);  // aux 2: non-temporal
+        }
+    } else {
+        const uint64_t g = base | local_to_global_low(tid, pd);
+#pragma unroll
+        for (int j = 0; j < 8; j++) tile[swz(j * NT + tid)] = load_nt<R>(&psi[g | pd.top[j]]);
     }
     __builtin_amdgcn_s_setprio(0);
 }
@@ -1398,7 +1405,8 @@ __device__ __forceinline__ void pass8_groups(unsigned char* tb, int tid, uint64_
         const int l0 = ins_bit(ins_bit(ins_bit(tid, s0), s1), s2);
         // slot of member m = swz(l0 | bits of m) = swz(l0) ^ swz(bits of m) (the swizzle is linear
         // over XOR): one XOR with a host-computed byte offset per member
-        const int sl = swz(l0) << 4;
+        constexpr int ES = (int)sizeof(V2<R>);  // bytes per amplitude
+        const int sl = swz(l0) * ES;
         V2<R> A[8], B[8];
 #pragma unroll
         for (int m = 0; m < 8; m++) A[m] = *reinterpret_cast<const V2<R>*>(tb + (sl ^ pd.goff[gi][m]));
@@ -1429,7 +1437,7 @@ __device__ __forceinline__ void pass8_groups(unsigned char* tb, int tid, uint64_
                 const int bit = msk ? px_bit(G0, msk, pc >> j) : (int)((pc >> j) & 1u);
                 px |= bit << pd.gbits[gi][j];
             }
-            sx = sl ^ (swz(px) << 4);
+            sx = sl ^ (swz(px) * ES);
         }
 #pragma unroll
         for (int m = 0; m < 8; m++) *reinterpret_cast<V2<R>*>(tb + (sx ^ pd.goff[gi][m])) = A[m];
@@ -1446,7 +1454,7 @@ __global__ void __launch_bounds__(1024) k_pass8(V2<R>* __restrict__ psi, const P
     const int tid = threadIdx.x, NT = blockDim.x;  // NT = 2^b / 8: one row of 8 per thread
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint64_t base = tile_base(blockIdx.x, pd);
-    pass8_load<R>(psi, tile, base, local_to_global_low(swz(tid), pd), pd, NT, wave);
+    pass8_load<R>(psi, tile, base, tid, pd, NT, wave);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     pass8_groups<R>(lds_raw, tid, base, pd, ops, runs);
@@ -1651,7 +1659,7 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
                 int c = 0;
                 for (int j = 0; j < 3; j++)
                     if ((m >> j) & 1) c |= 1 << pd.gbits[gi][j];
-                pd.goff[gi][m] = hswz(c) << 4;
+                pd.goff[gi][m] = hswz(c) * (int)sizeof(V2<R>);
             }
             int pxc = 0;
             bool lane_px = false;
@@ -1659,7 +1667,7 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
                 if (G.pxm[j]) lane_px = true;
                 if ((G.pxc >> j) & 1u) pxc |= 1 << pd.gbits[gi][j];
             }
-            pd.gpxoff[gi] = hswz(pxc) << 4;
+            pd.gpxoff[gi] = hswz(pxc) * (int)sizeof(V2<R>);
             if (lane_px) {
                 pd.pxlane_mask |= 1u << gi;
                 pd.g0_mask |= 1u << gi;
@@ -1815,7 +1823,7 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
         const uint64_t tiles = 1ull << (s.n - pd.b);
         const unsigned threads = (unsigned)std::min(512, std::max(64, (1 << pd.b) >> 4));  // one row each
         if (it.pass.group_bits == 3) {
-            if constexpr (NS == 1 && sizeof(R) == 8) {
+            if constexpr (NS == 1) {
                 // the load swizzle needs at least 128 threads (slot bits 0-6 from the thread index)
                 if (pd.b < 10 || pd.b > 13) throw Error(QFVM_ERR_RUNTIME, "row-of-8 tiles hold 2^10 to 2^13 amplitudes");
                 const size_t lds8 = ((size_t)1 << pd.b) * sizeof(V2<R>);
@@ -1827,7 +1835,7 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
                 QFVM_LAUNCHED();
                 continue;
             } else {
-                throw Error(QFVM_ERR_RUNTIME, "row-of-8 passes are one-state complex128 passes");
+                throw Error(QFVM_ERR_RUNTIME, "row-of-8 passes are one-state passes");
             }
         }
         // per-tile marker partials live in the scratch buffer (fetched per pass: a dense gate

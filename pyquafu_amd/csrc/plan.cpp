@@ -403,9 +403,10 @@ static FuseConfig config_of(const State& s, int states) {
     cfg.low_bits = s.low_bits > 0 ? s.low_bits : default_low_bits(s.dtype, states);
     cfg.max_groups = std::max(1, std::min(s.max_groups, FUSE_MAX_GROUPS));
     cfg.max_cost = s.pass_cost;
-    // complex128 one-state passes run on the row-of-8 kernel (3-qubit groups) when its tile fits
+    // one-state passes run on the row-of-8 kernel (3-qubit groups) when its tile fits: complex128,
+    // and complex64 with option "rows8_c64"
     const int b = std::min(cfg.tile_bits, s.n);
-    if (states == 1 && s.dtype == QFVM_C128 && s.rows8 && b >= 10 && b <= 13) cfg.group_bits = 3;
+    if (states == 1 && s.rows8 && (s.dtype == QFVM_C128 || s.rows8_c64) && b >= 10 && b <= 13) cfg.group_bits = 3;
     return cfg;
 }
 

@@ -394,7 +394,8 @@ def test_errors_are_reported():
 # kernel variants of complex128 one-state passes: rows of 8 with 3-qubit groups (the default for
 # 2^10 .. 2^13 tiles), split rows of 16, one lane per row of 16, and the persistent prefetching kernel
 KERNELS = {"rows8": (("rows8", 1),), "split": (("rows8", 0), ("split_rows", 1)),
-           "classic": (("rows8", 0), ("split_rows", 0)), "stream": (("rows8", 0), ("split_rows", 0), ("stream", 1))}
+           "classic": (("rows8", 0), ("split_rows", 0)), "stream": (("rows8", 0), ("split_rows", 0), ("stream", 1)),
+           "rows8c64": (("rows8", 1), ("rows8_c64", 1))}
 
 
 @pytest.mark.parametrize("dtype,tile,low,groups,kernel", [
@@ -405,7 +406,8 @@ KERNELS = {"rows8": (("rows8", 1),), "split": (("rows8", 0), ("split_rows", 1)),
     ("c128", 10, 5, 12, "split"), ("c128", 10, 6, 3, "split"), ("c128", 10, 3, 24, "split"),
     ("c64", 12, 5, 12, "split"), ("c128", 11, 5, 12, "rows8"), ("c128", 10, 3, 4, "rows8"),
     ("c128", 10, 6, 24, "rows8"), ("c128", 12, 4, 2, "rows8"), ("c128", 13, 5, 12, "rows8"),
-    ("c128", 11, 2, 24, "rows8"), ("c128", 12, 6, 12, "rows8")])
+    ("c128", 11, 2, 24, "rows8"), ("c128", 12, 6, 12, "rows8"), ("c64", 12, 5, 12, "rows8c64"),
+    ("c64", 10, 3, 4, "rows8c64"), ("c64", 13, 6, 24, "rows8c64"), ("c64", 11, 2, 24, "rows8c64")])
 def test_fused_matches_oracle(dtype, tile, low, groups, kernel):
     rng = np.random.Generator(np.random.PCG64(tile * 100 + low))
     n = 17
@@ -521,7 +523,8 @@ def diagonal_heavy_program(n: int, n_ops: int, rng):
 
 @pytest.mark.parametrize("n,dtype,kernel", [(9, "c128", "split"), (13, "c128", "split"), (16, "c128", "split"),
                                             (13, "c128", "classic"), (16, "c128", "stream"), (13, "c64", "split"),
-                                            (10, "c128", "rows8"), (13, "c128", "rows8"), (16, "c128", "rows8")])
+                                            (10, "c128", "rows8"), (13, "c128", "rows8"), (16, "c128", "rows8"),
+                                            (16, "c64", "rows8c64")])
 def test_diagonal_runs_match_oracle(n, dtype, kernel):
     rng = np.random.Generator(np.random.PCG64(1900 + n))
     prog = diagonal_heavy_program(n, 400, rng)
