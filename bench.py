@@ -365,42 +365,47 @@ def batch_leg(count: int, workers: int, graphs: bool = True, steps: int = 3, pin
 
 
 def dense_leg(n: int = 28, reps: int = 3):
-    """Dense complex128 gates on k = 4, 5 and 6 targets (seeded Haar-random unitaries, PCG64 seed
-    20250006; targets the k lowest, the k highest, or qubits 8 .. 8+k-1) on an n-qubit state, on
-    the FP64 matrix cores (option mfma = 1) and on the vector-ALU kernels (mfma = 0). Per gate:
-    mean HIP-event duration, algorithmic bytes 2 * 16 * 2^n and flops 8 * 2^k * 2^n."""
+    """Dense gates on k = 4, 5 and 6 targets (seeded Haar-random unitaries, PCG64 seed 20250006;
+    targets the k lowest, the k highest, or qubits 8 .. 8+k-1) on an n-qubit state, complex128 and
+    complex64, on the matrix cores (option mfma = 1: FP64 / FP32 MFMA) and on the vector-ALU kernels
+    (mfma = 0). Per gate: mean HIP-event duration, algorithmic bytes 2 * s * 2^n and flops
+    8 * 2^k * 2^n."""
     from pyquafu_amd import _native
     from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
 
     rng = np.random.Generator(np.random.PCG64(20250006))
-    dev = _native.DeviceState(n, "c128")
-    dev.set_profiling(True)
-    out = {"workload": f"{n}q c128, one dense k-target gate", "rows": []}
+    out = {"workload": f"{n}q, one dense k-target gate", "rows": []}
+    mats = {}
     for k in (4, 5, 6):
         d = 1 << k
         a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
         q, r = np.linalg.qr(a)
-        u = q * (np.diag(r) / np.abs(np.diag(r)))
-        for where, tg in (("low", list(range(k))), ("mid", list(range(8, 8 + k))), ("high", list(range(n - k, n)))):
-            b = ProgramBuilder()
-            b.add(MULTI, tg, matrix=u)
-            prog = b.program(n_qubits=n)
-            for mfma in (1, 0):
-                dev.set_option("mfma", mfma)
-                dev.init_zero()
-                dev.apply(prog)
-                dev.synchronize()
-                dev.profile(reset=True)
-                for _ in range(reps):
+        mats[k] = q * (np.diag(r) / np.abs(np.diag(r)))
+    for dtype, amp in (("c128", 16), ("c64", 8)):
+        dev = _native.DeviceState(n, dtype)
+        dev.set_profiling(True)
+        for k in (4, 5, 6):
+            d = 1 << k
+            for where, tg in (("low", list(range(k))), ("mid", list(range(8, 8 + k))), ("high", list(range(n - k, n)))):
+                b = ProgramBuilder()
+                b.add(MULTI, tg, matrix=mats[k])
+                prog = b.program(n_qubits=n)
+                for mfma in (1, 0):
+                    dev.set_option("mfma", mfma)
+                    dev.init_zero()
                     dev.apply(prog)
-                dev.synchronize()
-                prof = dev.profile(reset=True)
-                name = "multi_mfma" if mfma else ("multi_reg" if k <= 5 else "multi_lds")
-                ms = prof[name]["ms"] / prof[name]["launches"]
-                out["rows"].append({"k": k, "targets": where, "kernel": name, "ms": round(ms, 3),
-                                    "GBps": round(2 * 16 * 2 ** n / ms / 1e6, 1),
-                                    "TFLOPs": round(8 * d * 2 ** n / ms / 1e9, 2)})
-    dev.close()
+                    dev.synchronize()
+                    dev.profile(reset=True)
+                    for _ in range(reps):
+                        dev.apply(prog)
+                    dev.synchronize()
+                    prof = dev.profile(reset=True)
+                    name = "multi_mfma" if mfma else ("multi_reg" if k <= 5 else "multi_lds")
+                    ms = prof[name]["ms"] / prof[name]["launches"]
+                    out["rows"].append({"dtype": dtype, "k": k, "targets": where, "kernel": name, "ms": round(ms, 3),
+                                        "GBps": round(2 * amp * 2 ** n / ms / 1e6, 1),
+                                        "TFLOPs": round(8 * d * 2 ** n / ms / 1e9, 2)})
+        dev.close()
     return out
 
 

@@ -96,47 +96,86 @@ __global__ void __launch_bounds__(256) k_multi_reg(V2<R>* __restrict__ psi, uint
     }
 }
 
-// Dense 2^K x 2^K complex128 matrix on the FP64 matrix cores (K = 4, 5, 6). The gate is a real GEMM:
-// with X the tile's amplitudes as columns (rows 0..D-1 real parts, D..2D-1 imaginary parts) and
+// Dense 2^K x 2^K complex matrix on the matrix cores (K = 4, 5, 6): complex128 on
+// v_mfma_f64_16x16x4f64, complex64 on v_mfma_f32_16x16x4f32. The gate is a real GEMM: with X the
+// tile's amplitudes as columns (rows 0..D-1 real parts, D..2D-1 imaginary parts) and
 // A = [[Ur, -Ui], [Ui, Ur]], Y = A X. One workgroup (4 wavefronts) takes NG = 64 groups: gather
-// into LDS, the wavefronts multiply with v_mfma_f64_16x16x4f64 (A held in registers, X from LDS),
-// results back through LDS, scatter. A dense gate moves 2 x 16 B per amplitude and costs 8 x 2^K
-// flops per amplitude: 4, 8 and 16 flop/B for K = 4, 5, 6 against the FP64 ridge of ~9.8 flop/B,
-// so from K = 5 on the arithmetic decides and the matrix cores carry it.
-// Operand layouts of v_mfma_f64_16x16x4f64 (one 16x16 output block per wavefront):
+// into LDS, the wavefronts multiply (A held in registers, X from LDS), results back through LDS,
+// scatter. A dense gate moves 2 x s B per amplitude and costs 8 x 2^K flops per amplitude: at
+// complex128 4, 8 and 16 flop/B for K = 4, 5, 6 against the FP64 ridge of ~9.8 flop/B, so from
+// K = 5 on the arithmetic decides and the matrix cores carry it.
+// Operand layouts of both instructions (one 16x16 output block per wavefront):
 //   A (16 x 4): lane l supplies A[l % 16][l / 16];  B (4 x 16): lane l supplies B[l / 16][l % 16];
-//   D (16 x 16): lane l holds D[4 v + l / 16][l % 16] in accumulator element v = 0..3
-//   (measured on gfx950 with one-hot operands).
-typedef double dx4 __attribute__((ext_vector_type(4)));
+//   D (16 x 16): lane l holds, in accumulator element v = 0..3, D[4 v + l / 16][l % 16] for the f64
+//   instruction (measured on gfx950 with one-hot operands) and D[4 (l / 16) + v][l % 16] for f32.
+template <typename R>
+struct MfmaOp;
+template <>
+struct MfmaOp<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int kq) { return 4 * v + kq; }
+};
+template <>
+struct MfmaOp<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int kq) { return 4 * kq + v; }
+};
 constexpr int MFMA_NG = 64;  // groups per workgroup tile
-// LDS row pitch in doubles: odd, so that the member-fast gather (consecutive lanes on consecutive
-// rows) hits 32 distinct bank pairs; 79 keeps two K = 6 workgroups within the 160 KiB of a CU
+// LDS row pitch in elements: odd, so that the member-fast gather (consecutive lanes on consecutive
+// rows) hits distinct banks; 79 keeps two K = 6 complex128 workgroups within the 160 KiB of a CU
 constexpr int mfma_pitch(int K) { return K == 6 ? 79 : 81; }
+// LDS bytes of one workgroup: the M x P tile, plus (complex64) the group-base and member-offset
+// tables, which complex128 keeps in the row padding
+template <typename R, int K>
+constexpr size_t mfma_lds() {
+    return (size_t)(2 << K) * mfma_pitch(K) * sizeof(R) + (sizeof(R) == 8 ? 0 : (MFMA_NG + (1 << K)) * sizeof(uint64_t));
+}
 
 // Persistent: each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ... For K >= 5 wavefront
 // w owns the row blocks w, w + 4, .. of A (M / 64 of them) and all four 16-column blocks; for K = 4
 // (two row blocks) it owns both row blocks and column block w. Its A fragments stay in registers for
 // the whole launch, so per tile only X comes from LDS (one fragment per column block and k step).
-template <int K>
-__global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
-                                                    uint64_t cmask, TMask<K> tm, const double* __restrict__ at,
-                                                    int member_fast, uint64_t tiles) {
+template <typename R, int K>
+__global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(V2<R>* __restrict__ psi, uint64_t ngroups,
+                                                                    InsertSpec sp, uint64_t cmask, TMask<K> tm,
+                                                                    const R* __restrict__ at, int member_fast,
+                                                                    uint64_t tiles) {
+    typedef typename MfmaOp<R>::v4 v4;
     constexpr int D = 1 << K, M = 2 * D, P = mfma_pitch(K), KS = M / 4;
     constexpr bool ROWSPLIT = M >= 64;
     constexpr int NR = ROWSPLIT ? M / 64 : M / 16, NC = ROWSPLIT ? 4 : 1;  // blocks per wavefront
-    // M x P doubles; the pitch padding (PAD slots per row) of the first rows holds the tile's 64
-    // group bases, that of the next rows the D member offsets, so gather and scatter look both up
+    // complex128: M x P doubles; the pitch padding (PAD slots per row) of the first rows holds the
+    // tile's 64 group bases, that of the next rows the D member offsets. complex64: the tables
+    // follow the M x P floats.
     constexpr int PAD = P - MFMA_NG, GROWS = (MFMA_NG + PAD - 1) / PAD;
     static_assert(GROWS + (D + PAD - 1) / PAD <= M, "lookup tables exceed the padding");
-    extern __shared__ double xs[];
-    uint64_t* const xw = reinterpret_cast<uint64_t*>(xs);
-    auto gbase = [&](int c) -> uint64_t& { return xw[(c / PAD) * P + MFMA_NG + c % PAD]; };
-    auto moff = [&](int m) -> uint64_t& { return xw[(GROWS + m / PAD) * P + MFMA_NG + m % PAD]; };
+    extern __shared__ unsigned char lds_raw[];
+    R* const xs = reinterpret_cast<R*>(lds_raw);
+    uint64_t* const xw = reinterpret_cast<uint64_t*>(lds_raw);
+    uint64_t* const tw = reinterpret_cast<uint64_t*>(lds_raw + (size_t)M * P * sizeof(R));
+    auto gbase = [&](int c) -> uint64_t& {
+        if constexpr (sizeof(R) == 8)
+            return xw[(c / PAD) * P + MFMA_NG + c % PAD];
+        else
+            return tw[c];
+    };
+    auto moff = [&](int m) -> uint64_t& {
+        if constexpr (sizeof(R) == 8)
+            return xw[(GROWS + m / PAD) * P + MFMA_NG + m % PAD];
+        else
+            return tw[MFMA_NG + m];
+    };
     constexpr int ITER = D * MFMA_NG / 256;  // elements per thread and tile (4, 8 or 16)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, l16 = lane & 15;
     auto rowb = [&](int j) { return ROWSPLIT ? wave + 4 * j : j; };
     auto colb = [&](int c) { return ROWSPLIT ? c : wave; };
-    double a[NR][KS];  // A[rowb(j) 16 + l16][4 ks + kq]
+    R a[NR][KS];  // A[rowb(j) 16 + l16][4 ks + kq]
 #pragma unroll
     for (int j = 0; j < NR; j++)
 #pragma unroll
@@ -164,13 +203,13 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __r
         constexpr int CH = K == 6 ? 4 : (ITER < 8 ? ITER : 8);
 #pragma unroll
         for (int i0 = 0; i0 < ITER; i0 += CH) {
-            double2 v[CH];
+            V2<R> v[CH];
 #pragma unroll
             for (int i = 0; i < CH; i++) {
                 int m, c;
                 where(tid + 256 * (i0 + i), m, c);
                 const uint64_t gb = gbase(c);
-                v[i] = gb != ~0ull ? psi[gb | moff(m)] : make_double2(0.0, 0.0);
+                v[i] = gb != ~0ull ? psi[gb | moff(m)] : mk<R>(0, 0);
             }
 #pragma unroll
             for (int i = 0; i < CH; i++) {
@@ -181,21 +220,20 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __r
             }
         }
         __syncthreads();
-        dx4 acc[NR][NC];
+        v4 acc[NR][NC];
 #pragma unroll
         for (int j = 0; j < NR; j++)
 #pragma unroll
-            for (int ct = 0; ct < NC; ct++) acc[j][ct] = dx4{0.0, 0.0, 0.0, 0.0};
+            for (int ct = 0; ct < NC; ct++) acc[j][ct] = v4{0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            double b[NC];
+            R b[NC];
 #pragma unroll
             for (int ct = 0; ct < NC; ct++) b[ct] = xs[(4 * ks + kq) * P + colb(ct) * 16 + l16];
 #pragma unroll
             for (int j = 0; j < NR; j++)
 #pragma unroll
-                for (int ct = 0; ct < NC; ct++)
-                    acc[j][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j][ks], b[ct], acc[j][ct], 0, 0, 0);
+                for (int ct = 0; ct < NC; ct++) acc[j][ct] = MfmaOp<R>::mma(a[j][ks], b[ct], acc[j][ct]);
         }
         __syncthreads();  // every wavefront has read X
 #pragma unroll
@@ -203,14 +241,15 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(double2* __r
 #pragma unroll
             for (int ct = 0; ct < NC; ct++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) xs[(rowb(j) * 16 + 4 * v + kq) * P + colb(ct) * 16 + l16] = acc[j][ct][v];
+                for (int v = 0; v < 4; v++)
+                    xs[(rowb(j) * 16 + MfmaOp<R>::row(v, kq)) * P + colb(ct) * 16 + l16] = acc[j][ct][v];
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < ITER; i++) {
             int m, c;
             where(tid + 256 * i, m, c);
             const uint64_t gb = gbase(c);
-            if (gb != ~0ull) psi[gb | moff(m)] = make_double2(xs[m * P + c], xs[(m + D) * P + c]);
+            if (gb != ~0ull) psi[gb | moff(m)] = mk<R>(xs[m * P + c], xs[(m + D) * P + c]);
         }
         __syncthreads();  // the next tile rewrites xs and gbase
     }
@@ -366,7 +405,7 @@ static void multi_reg_launch(State& s, const Gate& g, const InsertSpec& sp, uint
     QFVM_LAUNCHED();
 }
 
-template <int K>
+template <typename R, int K>
 static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uint64_t ngroups) {
     constexpr int D = 1 << K, M = 2 * D;
     TMask<K> tm{};
@@ -376,34 +415,35 @@ static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uin
             if ((m >> b) & 1) o |= 1ull << g.targets[b];
         tm.m[m] = o;
     }
-    // A^T of the real embedding, at[k * M + r] = A[r][k], staged through scratch
-    std::vector<double> at((size_t)M * M);
+    // A^T of the real embedding, at[k * M + r] = A[r][k], in the state's precision, staged through scratch
+    std::vector<R> at((size_t)M * M);
     for (int r = 0; r < D; r++)
         for (int q = 0; q < D; q++) {
             const cd u = g.mk[(size_t)r * D + q];
-            at[(size_t)q * M + r] = u.real();
-            at[(size_t)(q + D) * M + r] = -u.imag();
-            at[(size_t)q * M + r + D] = u.imag();
-            at[(size_t)(q + D) * M + r + D] = u.real();
+            at[(size_t)q * M + r] = (R)u.real();
+            at[(size_t)(q + D) * M + r] = (R)-u.imag();
+            at[(size_t)q * M + r + D] = (R)u.imag();
+            at[(size_t)(q + D) * M + r + D] = (R)u.real();
         }
-    double* dat = reinterpret_cast<double*>(s.ensure_scratch(at.size() * sizeof(double)));
-    QFVM_HIP(hipMemcpyAsync(dat, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    R* dat = reinterpret_cast<R*>(s.ensure_scratch(at.size() * sizeof(R)));
+    QFVM_HIP(hipMemcpyAsync(dat, at.data(), at.size() * sizeof(R), hipMemcpyHostToDevice, s.stream));
     int lowest = 64;
     for (int t : g.targets) lowest = std::min(lowest, t);
     int lowest_free = 0;  // lowest qubit outside the gate
     while (lowest_free < s.n && (((g.cmask >> lowest_free) & 1) || std::find(g.targets.begin(), g.targets.end(), lowest_free) != g.targets.end()))
         lowest_free++;
     const int member_fast = lowest < lowest_free ? 1 : 0;
-    const size_t lds = (size_t)M * mfma_pitch(K) * sizeof(double);
-    if (lds > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_multi_mfma<K>), s.device);
+    const size_t lds = mfma_lds<R, K>();
+    const void* fn = reinterpret_cast<const void*>(&k_multi_mfma<R, K>);
+    if (lds > 65536) allow_large_lds(fn, s.device);
     const uint64_t tiles = (ngroups + MFMA_NG - 1) / MFMA_NG;
     int cus = 0, per_cu = 0;  // resident workgroups: registers and LDS both count
     QFVM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device));
-    QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_multi_mfma<K>), 256, lds));
+    QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1));
-    KernelScope ks(s, "multi_mfma", (double)ngroups * D * 2.0 * sizeof(double2));
-    hipLaunchKernelGGL((k_multi_mfma<K>), dim3(grid), dim3(256), lds, s.stream,
-                       reinterpret_cast<double2*>(s.psi), ngroups, sp, g.cmask, tm, dat, member_fast, tiles);
+    KernelScope ks(s, "multi_mfma", (double)ngroups * D * 2.0 * sizeof(V2<R>));
+    hipLaunchKernelGGL((k_multi_mfma<R, K>), dim3(grid), dim3(256), lds, s.stream,
+                       reinterpret_cast<V2<R>*>(s.psi), ngroups, sp, g.cmask, tm, dat, member_fast, tiles);
     QFVM_LAUNCHED();
     QFVM_HIP(hipStreamSynchronize(s.stream));  // the host matrix copy must outlive the async copy
 }
@@ -418,16 +458,14 @@ static void multi_typed(State& s, const Gate& g) {
     pos.insert(pos.end(), g.targets.begin(), g.targets.end());
     const InsertSpec sp = make_spec(pos);
     const uint64_t ngroups = s.size >> pos.size();
-    if constexpr (sizeof(R) == 8) {
-        if (s.mfma && k >= 4 && k <= 6) {
-            if (k == 4)
-                multi_mfma_launch<4>(s, g, sp, ngroups);
-            else if (k == 5)
-                multi_mfma_launch<5>(s, g, sp, ngroups);
-            else
-                multi_mfma_launch<6>(s, g, sp, ngroups);
-            return;
-        }
+    if (s.mfma && k >= 4 && k <= 6) {
+        if (k == 4)
+            multi_mfma_launch<R, 4>(s, g, sp, ngroups);
+        else if (k == 5)
+            multi_mfma_launch<R, 5>(s, g, sp, ngroups);
+        else
+            multi_mfma_launch<R, 6>(s, g, sp, ngroups);
+        return;
     }
     // matrix -> device (scratch), converted to the state's precision
     std::vector<V2<R>> hm(dim * dim);

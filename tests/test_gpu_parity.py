@@ -611,12 +611,14 @@ def test_full_size_round_trip_30_qubits():
 
 
 # ------------------------------------------------------------------ dense gates on the matrix cores
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("k,where,nc", [(4, "low", 0), (4, "mixed", 1), (5, "low", 0), (5, "high", 1),
                                         (5, "mixed", 2), (6, "low", 1), (6, "high", 0), (6, "mixed", 2)])
-def test_dense_mfma_matches_oracle(k, where, nc):
-    """complex128 dense gates on 4 to 6 targets run on v_mfma_f64_16x16x4f64 (option mfma, the
-    default); against the oracle and against the VALU kernels (mfma = 0), with the targets among
-    the lowest qubits (member-fast tile order), the highest, or spread, and with controls."""
+def test_dense_mfma_matches_oracle(k, where, nc, dtype):
+    """Dense gates on 4 to 6 targets run on the matrix cores (option mfma, the default:
+    v_mfma_f64_16x16x4f64 at complex128, v_mfma_f32_16x16x4f32 at complex64); against the oracle
+    and against the VALU kernels (mfma = 0), with the targets among the lowest qubits (member-fast
+    tile order), the highest, or spread, and with controls."""
     from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
     from tests.progs import random_unitary
 
@@ -641,12 +643,12 @@ def test_dense_mfma_matches_oracle(k, where, nc):
     want = cpu.store()
     got = {}
     for mfma in (1, 0):
-        dev = gpu_state(n)
+        dev = gpu_state(n, dtype)
         dev.set_option("mfma", mfma)
         dev.load(psi0)
         dev.apply(prog)
         got[mfma] = dev.store()
         prof = dev.profile()
         assert ("multi_mfma" in prof) == bool(mfma)
-    assert np.abs(got[1] - want).max() < TOL["c128"]
-    assert np.abs(got[1] - got[0]).max() < TOL["c128"]
+    assert np.abs(got[1] - want).max() < TOL[dtype]
+    assert np.abs(got[1] - got[0]).max() < TOL[dtype]
