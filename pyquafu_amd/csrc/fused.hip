@@ -428,6 +428,18 @@ __device__ void grad_stage8(const V2<R>* phi, const V2<R>* lam, uint64_t g_tid, 
         const V2<R> p = phi[swz(e)], l = lam[swz(e)];
         w[j] = (double)l.x * p.y - (double)l.y * p.x;
     }
+    // An uncontrolled Z marker's sum over the thread's elements, in the element order of the
+    // general loop below (so the same roundings): on one of the tile's three top bits (bit kk of
+    // the element number j), wt[kk] = sum of (bit kk of j ? -w : w); on any other qubit the sign is
+    // the lane's own index bit, the same for all 8 elements: +-wsum.
+    double wsum = 0.0, wt[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        wsum += w[j];
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) wt[kk] += ((j >> kk) & 1) ? -w[j] : w[j];
+    }
+    const int top0 = pd.q[pd.b - 3], top1 = pd.q[pd.b - 2], top2 = pd.q[pd.b - 1];
     for (int c0 = 0; c0 < cnt; c0 += 8) {
         const int cn = min(8, cnt - c0);
         double acc[8];
@@ -441,12 +453,33 @@ __device__ void grad_stage8(const V2<R>* phi, const V2<R>* lam, uint64_t g_tid, 
             uint64_t gk = g_tid;
             asm volatile("" : "+v"(tk), "+v"(gk));
             double a = 0.0;
-            if (o.g == GEN_Z) {
+            if (o.g == GEN_Z && o.cm_ext == 0) {
+                const int q = (int)o.cm_group;
+                a = q == top0 ? wt[0]
+                    : q == top1 ? wt[1]
+                    : q == top2 ? wt[2]
+                    : (((gk >> q) & 1ull) ? 0.0 - wsum : wsum);  // 0 - x: +0 for x = +0, as the loop gives
+            } else if (o.g == GEN_Z) {
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const uint64_t gi = gk | pd.top[j];
                     const double v = ((gi >> o.cm_group) & 1ull) ? -w[j] : w[j];
                     a += ((gi & o.cm_ext) == o.cm_ext) ? v : 0.0;
+                }
+            } else if (o.cm_ext == 0) {  // uncontrolled X / Y: no index tests
+                const int lb = (int)o.cm_group;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int e = j * NT + tk;
+                    const V2<R> l = lam[swz(e)], q = phi[swz(e ^ (1 << lb))];
+                    double v;
+                    if (o.g == GEN_X) {
+                        v = (double)l.x * q.y - (double)l.y * q.x;
+                    } else {
+                        const double re = (double)l.x * q.x + (double)l.y * q.y;
+                        v = ((e >> lb) & 1) ? re : -re;
+                    }
+                    a += v;
                 }
             } else {
                 const int lb = (int)o.cm_group;
