@@ -8,7 +8,8 @@ Printed (one JSON line):
   value          HBM GB/s in the pass model: sum over kernel launches of the bytes each launch must
                  read + write (2 * 16 B * amplitudes it touches) / wall time of the step
   roofline       the dominant kernel's algorithmic bytes per launch / its mean HIP-event duration,
-                 against MI355X's 8 TB/s HBM peak
+                 against MI355X's 8 TB/s HBM peak; stream_ref times a plain in-place streaming kernel
+                 (torch neg_) over a buffer of the state's size, and frac_of_stream = achieved / its rate
   gate_equiv     the unfused per-gate byte model (sum over gates of 2 * 16 B * 2^(n - controls)) /
                  step time: comparable with the CPU, may exceed the HBM peak once gates are fused
   cpu_baseline   the CPU oracle (a restatement of qfvm's OpenMP engine, built with its flags) on
@@ -143,6 +144,30 @@ def copy_calibration() -> int:
     print(json.dumps({"copy_calibration": {"op": "torch.neg float64", "read_bytes_per_launch": nbytes,
                                            "write_bytes_per_launch": nbytes, "launches": 3}}), flush=True)
     return 0
+
+
+def stream_reference(nbytes: int) -> dict:
+    """A plain streaming kernel over a buffer the size of the state: torch's in-place neg_ on
+    float64 (reads and writes every byte once), timed with events on torch's current stream, best of
+    3. Its rate is the practical HBM ceiling a pass is compared with."""
+    import torch
+
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    x.neg_()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(3):
+        a.record()
+        x.neg_()
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b))
+    del x
+    torch.cuda.empty_cache()
+    gbps = 2 * nbytes / (best / 1e3) / 1e9
+    return {"op": "torch neg_ in place, float64", "bytes": 2 * nbytes, "ms": round(best, 4), "GBps": round(gbps, 2),
+            "frac": round(gbps / PEAK_GBPS, 4)}
 
 
 def adjoint_leg(n: int, layers: int, fd_params: int = 8):
@@ -559,6 +584,10 @@ def main(argv=None) -> int:
         "expect_tfim": expect_tfim,
     }
     dev.close()
+    print("[bench] streaming reference", file=sys.stderr, flush=True)
+    sref = stream_reference(amp * (1 << n))
+    out["roofline"]["stream_ref"] = sref
+    out["roofline"]["frac_of_stream"] = round(achieved / sref["GBps"], 4)
     if not args.no_adjoint:
         print(f"[bench] adjoint leg: {args.adjoint_n}q, {args.adjoint_layers} layers", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(args.adjoint_n, args.adjoint_layers)
