@@ -10,6 +10,8 @@ Printed (one JSON line):
   roofline       the dominant kernel's algorithmic bytes per launch / its mean HIP-event duration,
                  against MI355X's 8 TB/s HBM peak; stream_ref times a plain in-place streaming kernel
                  (torch neg_) over a buffer of the state's size, and frac_of_stream = achieved / its rate
+  layout         the same circuit with the layout-aware planner (option "layout"): time per step,
+                 fused passes, speedup over the default planner
   gate_equiv     the unfused per-gate byte model (sum over gates of 2 * 16 B * 2^(n - controls)) /
                  step time: comparable with the CPU, may exceed the HBM peak once gates are fused
   cpu_baseline   the CPU oracle (a restatement of qfvm's OpenMP engine, built with its flags) on
@@ -543,6 +545,26 @@ def main(argv=None) -> int:
         "energy_h0.7": float(sum(tfim_vals[:n]) + 0.7 * sum(tfim_vals[n:])),
     }
 
+    # the layout-aware planner (option "layout") on the same circuit: fewer passes, more gates each
+    dev.set_option("layout", 1)
+    dev.apply(prog)
+    dev.synchronize()
+    dev.profile(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.apply(prog)
+    dev.synchronize()
+    dt_lay = (time.perf_counter() - t0) / args.steps
+    lay_prof = dev.profile(reset=True)
+    dev.set_option("layout", 0)
+    layout_leg = {
+        "option": "layout",
+        "ms_per_step": round(dt_lay * 1e3, 3),
+        "fused_passes_per_step": lay_prof.get("fused_pass", {}).get("launches", 0) / args.steps,
+        "speedup_vs_default": round(dt / dt_lay, 4),
+        "pass_model_GBps": round(sum(v["bytes"] for v in lay_prof.values()) / args.steps / dt_lay / 1e9, 2),
+    }
+
     ge = gate_equiv_bytes(prog, n, amp)
     out = {
         "metric": METRIC,
@@ -582,6 +604,7 @@ def main(argv=None) -> int:
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in prof.items()},
         "norm_error": norm_err,
         "expect_tfim": expect_tfim,
+        "layout": layout_leg,
     }
     dev.close()
     print("[bench] streaming reference", file=sys.stderr, flush=True)

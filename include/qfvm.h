@@ -320,6 +320,11 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 8-amplitude row (3-qubit groups) per lane instead of split rows, 0 = split rows
  *   "rows8_c64"   1 = complex64 one-state passes of such tiles do the same (with "rows8" on),
  *                 0 = (default) 16-amplitude rows
+ *   "layout"      1 = layout-aware planning of those row-of-8 passes: a pass may store its tile with
+ *                 its qubits in other places (the tile's addresses stay the same), so that the qubits
+ *                 the next passes act on sit in the low positions every tile holds; fewer passes per
+ *                 circuit, and passes without gates at the end of each call restore the qubit order,
+ *                 so the state between calls is always in the standard order. 0 = (default) off
  *   "stream"      1 = persistent fused-pass kernel that prefetches the next tile (complex128,
  *                 one-state passes of 2^11..2^13-amplitude tiles, when split_rows does not apply),
  *                 0 = one workgroup per tile
@@ -343,13 +348,15 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
 
 /*
  * The execution plan qfvm_apply would run for `prog` on an n-qubit state, as JSON (host only, no
- * device needed; no counterpart in the reference). Fused passes list their tile qubits and, per
- * 4-qubit group, the ops in execution order: {"kind", "g", "cmg", "cme", "run", "m": [re, im] x 4}; other
- * items carry the gate. Lets the planner's reordering be checked without a GPU. Arguments of 0
- * pick the defaults, as for qfvm_set_option.
+ * device needed; no counterpart in the reference). Fused passes list their tile qubits (positions),
+ * "out_bit" (layout-aware plans: where each tile bit is stored; empty = in place) and, per group,
+ * the ops in execution order: {"kind", "g", "cmg", "cme", "run", "m": [re, im] x 4}; other items
+ * carry the gate on the positions it runs at. Lets the planner's reordering be checked without a
+ * GPU. Arguments of 0 pick the defaults, as for qfvm_set_option; layout = 1 plans as option
+ * "layout" does.
  */
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
-                   int32_t pass_cost,
+                   int32_t pass_cost, int32_t layout,
                    const qfvm_program* prog, char* buf, int64_t cap, int64_t* len);
 
 #ifdef __cplusplus

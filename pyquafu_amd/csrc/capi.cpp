@@ -590,7 +590,7 @@ static uint64_t adjoint_key(const State& s, const qfvm_program* prog, const int6
                             const int32_t* len, int64_t n_terms) {
     uint64_t h = 1469598103934665603ull;
     const int32_t opts[] = {s.n, s.dtype, s.mfma, s.fusion, s.tile_bits, s.low_bits, s.max_groups, s.pass_cost, s.split_rows,
-                            s.stream_tiles, s.stagger, s.stagger_shift, s.rows8, s.rows8_c64};
+                            s.stream_tiles, s.stagger, s.stagger_shift, s.rows8, s.rows8_c64, s.layout};
     h = fnv(h, opts, sizeof(opts));
     h = fnv(h, &prog->n_ops, sizeof(prog->n_ops));
     for (int64_t k = 0; k < prog->n_ops; k++) {
@@ -1140,6 +1140,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->rows8 = value != 0;
         } else if (k == "rows8_c64") {
             s->rows8_c64 = value != 0;
+        } else if (k == "layout") {
+            s->layout = value != 0;
         } else if (k == "plan_cache") {
             s->plan_cache_on = value != 0;
             if (!value) s->plan_cache.reset();
@@ -1165,7 +1167,7 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
 }
 
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
-                   int32_t pass_cost, const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
+                   int32_t pass_cost, int32_t layout, const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
     return guarded([&] {
         if (n < 0 || n > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
         check_program(prog);
@@ -1186,6 +1188,7 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
         // the group width qfvm_apply would use with default options (complex128: rows of 8)
         const int bb = std::min(cfg.tile_bits, n);
         if (tmp.dtype == QFVM_C128 && bb >= 10 && bb <= 13) cfg.group_bits = 3;
+        cfg.layout = layout != 0 && cfg.group_bits == 3;
         std::vector<PlanItem> plan = plan_gates(gates, n, cfg);
         std::string j = "{\"items\": [";
         char t[512];
@@ -1199,6 +1202,8 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
             if (it.fused) {
                 j += "{\"fused\": true, \"group_bits\": " + std::to_string(it.pass.group_bits) + ", \"Q\": [";
                 for (size_t k = 0; k < it.pass.Q.size(); k++) j += (k ? ", " : "") + std::to_string(it.pass.Q[k]);
+                j += "], \"out_bit\": [";
+                for (size_t k = 0; k < it.pass.out_bit.size(); k++) j += (k ? ", " : "") + std::to_string(it.pass.out_bit[k]);
                 j += "], \"groups\": [";
                 for (size_t g = 0; g < it.pass.groups.size(); g++) {
                     const FGroup& G = it.pass.groups[g];

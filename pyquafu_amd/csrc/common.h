@@ -149,6 +149,7 @@ struct State {
     int split_rows = 1;     // fused-pass kernel with two lanes per 16-amplitude row (complex128)
     int rows8 = 1;          // complex128 one-state passes: one lane per 8-amplitude row, 3-qubit groups
     int rows8_c64 = 0;      // the same for complex64
+    int layout = 0;         // layout-aware planning of row-of-8 one-state passes (option "layout")
     int stagger = 0;        // start delay of half the persistent workgroups, in 8128-cycle sleeps
     int stagger_shift = 0;  // which blockIdx bit picks the delayed half
     // adjoint checkpoints (option "checkpoints"): up to this many copies of the forward state,

@@ -81,6 +81,10 @@ struct FPass {
     std::vector<int> Q;          // tile qubits, ascending; Q[0..L-1] = 0..L-1
     std::vector<FGroup> groups;
     int group_bits = FUSE_GROUP_BITS;  // qubits per compute group (3: rows of 8, 4: rows of 16)
+    // layout-aware plans: the pass stores local bit k of its tile at local bit out_bit[k] (a
+    // permutation of 0..b-1; empty = in place). The tile's addresses do not change, only which
+    // qubit each of its address bits stands for.
+    std::vector<int> out_bit;
 };
 
 struct FuseConfig {
@@ -89,6 +93,7 @@ struct FuseConfig {
     int max_groups;
     int max_cost;    // compute budget of a pass in quarter dense-gate units (0: unlimited)
     int group_bits = FUSE_GROUP_BITS;  // qubits per compute group: 3 or 4
+    bool layout = false;  // layout-aware planning (row-of-8 one-state passes only, see plan.cpp)
 };
 
 // Default tile shape per precision and state count (states = 2: the adjoint's two-state sweep, both
@@ -103,8 +108,9 @@ inline int default_low_bits(int /*dtype*/, int states) { return states == 1 ? 5 
 struct PlanItem {
     bool fused;
     FPass pass;
-    Gate gate;
+    Gate gate;     // single gates: the gate on the qubit positions of the layout it runs in
     int src = -1;  // single gates: index of the gate in the planned list
+    std::vector<int> l2p;  // single gates in layout-aware plans: position of each qubit (empty = identity)
 };
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg);

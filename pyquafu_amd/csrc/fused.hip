@@ -48,6 +48,15 @@ struct PassDesc {
     int32_t goff[FUSE_MAX_GROUPS][8];
     int32_t gpxoff[FUSE_MAX_GROUPS];
     uint32_t g0_mask, pxlane_mask;
+    // layout-aware row-of-8 passes (FPass::out_bit): output element e of the tile takes input
+    // element Pinv(e) (Pinv the inverse bit permutation). Thread t stores output elements
+    // rho(t) + j * NT, rho(t) = XOR of rbit[k] over the bits k of t (a reordering of the wavefront's
+    // 64 elements per store, chosen so the LDS reads avoid bank conflicts), read from slot
+    // swz(XOR of pbit[k] = Pinv(rbit[k])) ^ pst[j]; permuted = 0 stores every element in place
+    uint32_t permuted;
+    int32_t pbit[FUSE_MAX_TILE_BITS];
+    int32_t rbit[FUSE_MAX_TILE_BITS];
+    int32_t pst[8];
 };
 
 // Diagonal-run record (fused.h): 16 host factors f0, then for each subset S its term range
@@ -1492,6 +1501,21 @@ __global__ void __launch_bounds__(1024) k_pass8(V2<R>* __restrict__ psi, const P
     __syncthreads();
     pass8_groups<R>(lds_raw, tid, base, pd, ops, runs);
     const uint64_t g_tid = base | local_to_global_low(tid, pd);
+    if (pd.permuted) {  // the tile's qubits change places: element e is written from slot swz(Pinv(e))
+        int pt = 0, rt = 0;
+#pragma unroll
+        for (int k = 0; k < FUSE_MAX_TILE_BITS - 3; k++)
+            if (k < pd.b - 3 && ((tid >> k) & 1)) {
+                pt ^= pd.pbit[k];
+                rt ^= pd.rbit[k];
+            }
+        const int st = swz(pt);
+        const uint64_t g_rt = base | local_to_global_low(rt, pd);
+        __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+        for (int j = 0; j < 8; j++) store_nt<R>(&psi[g_rt | pd.top[j]], tile[st ^ pd.pst[j]]);
+        return;
+    }
     __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int j = 0; j < 8; j++) store_nt<R>(&psi[g_tid | pd.top[j]], tile[swz(j * NT + tid)]);
@@ -1752,6 +1776,53 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
         }
     }
     pd.gop[pd.ngroups] = (int)h.size() - first;
+    if (!p.out_bit.empty()) {
+        // out_bit[k]: where input local bit k goes; the store needs the inverse
+        if ((int)p.out_bit.size() != b || gbn != 3) throw Error(QFVM_ERR_RUNTIME, "qubit permutation outside a row-of-8 pass");
+        int from[FUSE_MAX_TILE_BITS];
+        for (int k = 0; k < b; k++) from[k] = -1;
+        for (int k = 0; k < b; k++) {
+            const int o = p.out_bit[(size_t)k];
+            if (o < 0 || o >= b || from[o] >= 0) throw Error(QFVM_ERR_RUNTIME, "tile qubit permutation is not a permutation");
+            from[o] = k;
+        }
+        const auto hswz = [](int l) { return l ^ ((l >> 4) & 7); };
+        pd.permuted = 1;
+        // lane order of the stores: lane bits 0..3 take output bits (among 0..5, the wavefront's 64
+        // elements) whose input bits land on distinct LDS slot bits 0..3 after the swizzle (input
+        // bit f < 4 on slot bit f, 4..6 on f - 4), so 16 consecutive lanes read 16 different slots
+        // modulo 16; the other bits keep their order
+        int rho[FUSE_MAX_TILE_BITS];
+        for (int k = 0; k < b - 3; k++) rho[k] = k;
+        {
+            const auto proj = [](int f) { return f < 4 ? f : (f < 7 ? f - 4 : -1); };
+            bool used[6] = {false, false, false, false, false, false};
+            int covered = 0;
+            for (int i = 0; i < 4; i++) {
+                int pick = -1;
+                for (int t = 0; t < 6 && pick < 0; t++)
+                    if (!used[t] && proj(from[t]) >= 0 && !((covered >> proj(from[t])) & 1)) pick = t;
+                for (int t = 0; t < 6 && pick < 0; t++)
+                    if (!used[t]) pick = t;
+                rho[i] = pick;
+                used[pick] = true;
+                if (proj(from[pick]) >= 0) covered |= 1 << proj(from[pick]);
+            }
+            int i = 4;
+            for (int t = 0; t < 6; t++)
+                if (!used[t]) rho[i++] = t;
+        }
+        for (int t = 0; t < b - 3; t++) {
+            pd.rbit[t] = 1 << rho[t];
+            pd.pbit[t] = 1 << from[rho[t]];
+        }
+        for (int j = 0; j < 8; j++) {
+            int v = 0;
+            for (int i = 0; i < 3; i++)
+                if ((j >> i) & 1) v |= 1 << from[b - 3 + i];
+            pd.pst[j] = hswz(v);
+        }
+    }
     return pd;
 }
 

@@ -74,6 +74,28 @@ int gate_cost(const Gate& g, const GInfo& gi) {
     }
 }
 
+// a qubit mask / a gate moved to the positions l2p[q] of a layout (l2p empty: identity)
+uint64_t move_mask(uint64_t m, const std::vector<int>& l2p) {
+    if (l2p.empty()) return m;
+    uint64_t r = 0;
+    while (m) {
+        const int q = __builtin_ctzll(m);
+        m &= m - 1;
+        r |= 1ull << l2p[(size_t)q];
+    }
+    return r;
+}
+
+Gate move_gate(const Gate& g, const std::vector<int>& l2p) {
+    Gate t = g;
+    if (l2p.empty()) return t;
+    if (g.target >= 0 && g.target < (int)l2p.size()) t.target = l2p[(size_t)g.target];
+    t.cmask = move_mask(g.cmask, l2p);
+    for (int& c : t.ctrls) c = l2p[(size_t)c];
+    for (int& q : t.targets) q = l2p[(size_t)q];
+    return t;
+}
+
 // two gates (or a gate and a group) commute when no qubit is non-diagonal for one and touched by
 // the other
 inline bool commutes(const GInfo& g, const GroupB& G) { return !((g.N & G.A) || ((g.N | g.D) & G.N)); }
@@ -144,6 +166,19 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
     // gate always fits and every pass makes progress
     const int L = b < n ? std::min(cfg.low_bits, b - 1) : b;
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    // Layout-aware plans (cfg.layout, row-of-8 passes): qubit q sits at position phys[q] (inv is the
+    // inverse); gates are planned on positions. At the end of every pass the tile qubits with the
+    // nearest next non-diagonal use move into the L low positions, which every later tile holds, so
+    // the next pass needs fewer new positions; passes without gates restore the identity at the end.
+    const bool lay = cfg.layout && b < n && gb == 3;
+    std::vector<int> phys, inv;
+    if (lay) {
+        phys.resize((size_t)n);
+        inv.resize((size_t)n);
+        for (int q = 0; q < n; q++) phys[(size_t)q] = inv[(size_t)q] = q;
+    }
+    bool moved = false;  // phys differs from the identity
+    static const std::vector<int> kIdentity;
     std::vector<GInfo> info(gates.size());
     for (size_t i = 0; i < gates.size(); i++) info[i] = info_of(gates[i]);
     std::vector<char> done(gates.size(), 0);
@@ -152,8 +187,9 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
     while (true) {
         while (first < gates.size() && done[first]) first++;
         if (first >= gates.size()) break;
+        const std::vector<int>& cur = moved ? phys : kIdentity;
         if (info[first].multi) {
-            plan.push_back(PlanItem{false, FPass{}, gates[first], (int)first});
+            plan.push_back(PlanItem{false, FPass{}, move_gate(gates[first], cur), (int)first, cur});
             done[first] = 1;
             continue;
         }
@@ -163,7 +199,11 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
         int cost = 0;
         for (size_t i = first; i < gates.size(); i++) {
             if (done[i]) continue;
-            const GInfo& g = info[i];
+            GInfo g = info[i];
+            if (moved) {
+                g.N = move_mask(g.N, phys);
+                g.D = move_mask(g.D, phys);
+            }
             const uint64_t A = g.N | g.D;
             bool take = !((g.N & blockA) || (g.D & blockN)) && !g.multi;
             uint64_t newQ = Q | g.N;
@@ -227,13 +267,14 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                 fg.stage = true;
                 for (int j = 0; j < FUSE_GROUP_BITS; j++) fg.S[j] = -1;
                 for (int gi : G.gates) {
+                    const Gate tg = move_gate(gates[gi], cur);
                     FOpHost o{};
                     o.src = -1;
                     o.kind = F_GRAD;
-                    o.g = gates[gi].gen;
-                    o.cm_ext = gates[gi].cmask;
-                    o.target = gates[gi].target;
-                    o.slot = gates[gi].slot;
+                    o.g = tg.gen;
+                    o.cm_ext = tg.cmask;
+                    o.target = tg.target;
+                    o.slot = tg.slot;
                     fg.ops.push_back(o);
                 }
                 it.pass.groups.push_back(std::move(fg));
@@ -253,7 +294,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             uint32_t C = 0;
             auto pending = [&](int j) { return M[j] != 0 || ((C >> j) & 1); };
             for (int gi : G.gates) {
-                FOpHost o = convert(gates[gi], info[gi], fg.S);
+                FOpHost o = convert(move_gate(gates[gi], cur), info[gi], fg.S);
                 o.src = gi;
                 for (int j = 0; j < FUSE_GROUP_BITS; j++) {
                     if (((o.cm_group >> j) & 1) && pending(j)) {
@@ -291,8 +332,102 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             mark_diagonal_runs(fg);
             it.pass.groups.push_back(std::move(fg));
         }
+        if (lay) {
+            // the L tile qubits whose next non-diagonal use comes first go to the low positions
+            const std::vector<int>& Qv = it.pass.Q;
+            const int bb = (int)Qv.size();
+            std::vector<size_t> nxt((size_t)bb, gates.size());
+            int found = 0;
+            for (size_t i = first; i < gates.size() && found < bb; i++) {
+                if (done[i]) continue;
+                for (int k = 0; k < bb; k++)
+                    if (nxt[(size_t)k] == gates.size() && ((info[i].N >> inv[(size_t)Qv[(size_t)k]]) & 1)) {
+                        nxt[(size_t)k] = i;
+                        found++;
+                    }
+            }
+            std::vector<int> order((size_t)bb);
+            for (int k = 0; k < bb; k++) order[(size_t)k] = k;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nxt[(size_t)x] < nxt[(size_t)y]; });
+            std::vector<char> chosen((size_t)bb, 0);
+            for (int k = 0; k < L; k++) chosen[(size_t)order[(size_t)k]] = 1;
+            std::vector<int> out((size_t)bb);
+            for (int k = 0; k < bb; k++) out[(size_t)k] = k;
+            int lo = 0;
+            bool any = false;
+            for (int k = L; k < bb; k++) {
+                if (!chosen[(size_t)k]) continue;
+                while (chosen[(size_t)lo]) lo++;  // a low slot whose qubit is not chosen (one exists per mover)
+                out[(size_t)k] = lo;
+                out[(size_t)lo] = k;
+                lo++;
+                any = true;
+            }
+            if (any) {
+                std::vector<int> was((size_t)bb);
+                for (int k = 0; k < bb; k++) was[(size_t)k] = inv[(size_t)Qv[(size_t)k]];
+                for (int k = 0; k < bb; k++) {
+                    const int q = was[(size_t)k], p = Qv[(size_t)out[(size_t)k]];
+                    phys[(size_t)q] = p;
+                    inv[(size_t)p] = q;
+                }
+                it.pass.out_bit = out;
+                moved = true;
+            }
+        }
         plan.push_back(std::move(it));
     }
+    // restore the identity layout: passes without gates, each holding the low positions and up to
+    // b - L high positions along the permutation's cycles; a tile qubit whose own position is in the
+    // tile goes there (at least one more qubit reaches its position per pass)
+    for (int guard = 0; moved && guard <= n; guard++) {
+        bool ident = true;
+        for (int q = 0; q < n; q++) ident = ident && phys[(size_t)q] == q;
+        if (ident) break;
+        uint64_t Qm = L >= 64 ? ~0ull : ((1ull << L) - 1);
+        for (int p = L; p < n; p++) {
+            if (inv[(size_t)p] == p) continue;
+            const uint64_t add = (1ull << p) | (phys[(size_t)p] >= L ? 1ull << phys[(size_t)p] : 0ull);
+            if (popc(Qm | add) <= b) Qm |= add;
+        }
+        for (int q = 0; q < n && popc(Qm) < b; q++) Qm |= 1ull << q;
+        PlanItem it{true, FPass{}, Gate{}};
+        it.pass.group_bits = gb;
+        std::vector<int> loc((size_t)n, -1);
+        for (int q = 0; q < n; q++)
+            if ((Qm >> q) & 1) {
+                loc[(size_t)q] = (int)it.pass.Q.size();
+                it.pass.Q.push_back(q);
+            }
+        const int bb = (int)it.pass.Q.size();
+        std::vector<int> out((size_t)bb, -1), was((size_t)bb);
+        std::vector<char> used((size_t)bb, 0);
+        for (int k = 0; k < bb; k++) {
+            was[(size_t)k] = inv[(size_t)it.pass.Q[(size_t)k]];
+            const int home = loc[(size_t)was[(size_t)k]];
+            if (home >= 0) {
+                out[(size_t)k] = home;
+                used[(size_t)home] = 1;
+            }
+        }
+        int nx = 0;
+        for (int k = 0; k < bb; k++) {
+            if (out[(size_t)k] >= 0) continue;
+            while (used[(size_t)nx]) nx++;
+            out[(size_t)k] = nx;
+            used[(size_t)nx] = 1;
+        }
+        for (int k = 0; k < bb; k++) {
+            const int q = was[(size_t)k], p = it.pass.Q[(size_t)out[(size_t)k]];
+            phys[(size_t)q] = p;
+            inv[(size_t)p] = q;
+        }
+        it.pass.out_bit = out;
+        plan.push_back(std::move(it));
+    }
+    if (moved)
+        for (int q = 0; q < n; q++)
+            if (phys[(size_t)q] != q) throw Error(QFVM_ERR_RUNTIME, "layout-aware plan did not restore the qubit order");
     return plan;
 }
 
@@ -321,7 +456,7 @@ void plan_signature(const std::vector<Gate>& gates, int n, const FuseConfig& cfg
     sig.reserve(gates.size() * 5 + 2);
     sig.push_back((uint64_t)n | ((uint64_t)cfg.tile_bits << 8) | ((uint64_t)cfg.low_bits << 16) |
                   ((uint64_t)cfg.max_groups << 24) | ((uint64_t)cfg.group_bits << 32));
-    sig.push_back((uint64_t)(uint32_t)cfg.max_cost);
+    sig.push_back((uint64_t)(uint32_t)cfg.max_cost | ((uint64_t)cfg.layout << 32));
     for (const Gate& g : gates) {
         const GInfo gi = info_of(g);
         const uint64_t cls = (gi.diag ? 1u : 0u) | (gi.multi ? 2u : 0u) | (gi.grad ? 4u : 0u) |
@@ -342,7 +477,7 @@ void plan_signature(const std::vector<Gate>& gates, int n, const FuseConfig& cfg
 void refresh_plan(std::vector<PlanItem>& plan, const std::vector<Gate>& gates) {
     for (PlanItem& it : plan) {
         if (!it.fused) {
-            it.gate = gates[(size_t)it.src];
+            it.gate = move_gate(gates[(size_t)it.src], it.l2p);
             continue;
         }
         for (FGroup& fg : it.pass.groups) {
@@ -407,6 +542,7 @@ static FuseConfig config_of(const State& s, int states) {
     // and complex64 with option "rows8_c64"
     const int b = std::min(cfg.tile_bits, s.n);
     if (states == 1 && s.rows8 && (s.dtype == QFVM_C128 || s.rows8_c64) && b >= 10 && b <= 13) cfg.group_bits = 3;
+    cfg.layout = s.layout && cfg.group_bits == 3;
     return cfg;
 }
 

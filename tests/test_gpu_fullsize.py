@@ -1,8 +1,9 @@
 """GPU parity at the benchmark workloads' real shapes, through the C-ABI, against the CPU oracle.
 
 * the 30-qubit complex128 benchmark circuit (random U3 + CX brickwork + 15 CRZ per layer, PCG64
-  seed 20250002): its first four layers (296 gates) on the GPU and on the oracle (the restatement
-  of qfvm's OpenMP engine), every one of the 2^30 amplitudes compared, 1e-12 max-abs;
+  seed 20250002): its first four layers (296 gates) on the GPU, with the default and with the
+  layout-aware planner, and on the oracle (the restatement of qfvm's OpenMP engine), every one of
+  the 2^30 amplitudes compared, 1e-12 max-abs;
 * the 24-qubit, 8-layer hardware-efficient ansatz with the periodic TFIM: all 384 adjoint partials
   and <H> against the oracle's grad_adjoint (gradiant.py:144-173) at 1e-10, plus the torch
   autograd.Function path;
@@ -35,19 +36,23 @@ def test_benchmark_circuit_30q_first_layers_match_oracle():
     qc = workloads.random_u3_cx(n, layers, 15, seed=20250002, measure=False)
     prog = compile_circuit(qc)
     assert prog.n_ops == layers * (n + n // 2 + (n - 1) // 2 + 15)
-    print(f"[30q] {prog.n_ops} gates on the GPU", flush=True)
-    dev = _native.DeviceState(n, "c128")
-    try:
-        dev.apply(prog)
-        got = dev.store()
-    finally:
-        dev.close()
+    got = {}
+    for lay in (0, 1):  # the default planner and the layout-aware one (option "layout")
+        print(f"[30q] {prog.n_ops} gates on the GPU, layout {lay}", flush=True)
+        dev = _native.DeviceState(n, "c128")
+        try:
+            dev.set_option("layout", lay)
+            dev.apply(prog)
+            got[lay] = dev.store()
+        finally:
+            dev.close()
     print("[30q] oracle", flush=True)
     cpu = CpuState(n)
     cpu.apply(prog)
-    err = _max_abs_chunked(got, cpu.view())
-    print(f"[30q] max-abs over 2^30 amplitudes vs oracle: {err:.3e}", flush=True)
-    assert err < 1e-12
+    for lay in (0, 1):
+        err = _max_abs_chunked(got[lay], cpu.view())
+        print(f"[30q] layout {lay}: max-abs over 2^30 amplitudes vs oracle: {err:.3e}", flush=True)
+        assert err < 1e-12
     del cpu, got
 
 

@@ -395,7 +395,8 @@ def test_errors_are_reported():
 # 2^10 .. 2^13 tiles), split rows of 16, one lane per row of 16, and the persistent prefetching kernel
 KERNELS = {"rows8": (("rows8", 1),), "split": (("rows8", 0), ("split_rows", 1)),
            "classic": (("rows8", 0), ("split_rows", 0)), "stream": (("rows8", 0), ("split_rows", 0), ("stream", 1)),
-           "rows8c64": (("rows8", 1), ("rows8_c64", 1))}
+           "rows8c64": (("rows8", 1), ("rows8_c64", 1)), "layout": (("rows8", 1), ("layout", 1)),
+           "layoutc64": (("rows8", 1), ("rows8_c64", 1), ("layout", 1))}
 
 
 @pytest.mark.parametrize("dtype,tile,low,groups,kernel", [
@@ -407,7 +408,9 @@ KERNELS = {"rows8": (("rows8", 1),), "split": (("rows8", 0), ("split_rows", 1)),
     ("c64", 12, 5, 12, "split"), ("c128", 11, 5, 12, "rows8"), ("c128", 10, 3, 4, "rows8"),
     ("c128", 10, 6, 24, "rows8"), ("c128", 12, 4, 2, "rows8"), ("c128", 13, 5, 12, "rows8"),
     ("c128", 11, 2, 24, "rows8"), ("c128", 12, 6, 12, "rows8"), ("c64", 12, 5, 12, "rows8c64"),
-    ("c64", 10, 3, 4, "rows8c64"), ("c64", 13, 6, 24, "rows8c64"), ("c64", 11, 2, 24, "rows8c64")])
+    ("c64", 10, 3, 4, "rows8c64"), ("c64", 13, 6, 24, "rows8c64"), ("c64", 11, 2, 24, "rows8c64"),
+    ("c128", 11, 5, 12, "layout"), ("c128", 10, 3, 4, "layout"), ("c128", 13, 4, 24, "layout"),
+    ("c128", 12, 6, 2, "layout"), ("c64", 12, 5, 12, "layoutc64"), ("c64", 11, 2, 24, "layoutc64")])
 def test_fused_matches_oracle(dtype, tile, low, groups, kernel):
     rng = np.random.Generator(np.random.PCG64(tile * 100 + low))
     n = 17
@@ -425,7 +428,7 @@ def test_fused_matches_oracle(dtype, tile, low, groups, kernel):
     assert dev.profile()["fused_pass"]["launches"] > 0
 
 
-@pytest.mark.parametrize("kernel", ["rows8", "split", "classic", "stream"])
+@pytest.mark.parametrize("kernel", ["rows8", "split", "classic", "stream", "layout"])
 def test_fused_equals_unfused_at_24_qubits(kernel):
     """The two GPU paths on the HBM workload's structure at 24 qubits, and the oracle."""
     n = 24
@@ -445,6 +448,37 @@ def test_fused_equals_unfused_at_24_qubits(kernel):
     cpu = CpuState(n)
     cpu.apply(prog)
     assert np.abs(fa - cpu.store()).max() < 1e-12
+
+
+def test_layout_fewer_passes_and_standard_order_between_calls():
+    """Option "layout" on the brickwork structure at 22 qubits: fewer fused passes than the plain
+    planner, and every call leaves the state in the standard qubit order (two calls in a row, then
+    expectations and sampling on the result, against the oracle)."""
+    n = 22
+    prog = compile_circuit(workloads.random_u3_cx(n, 6, 8, seed=41, measure=False))
+    cpu = CpuState(n)
+    cpu.apply(prog)
+    cpu.apply(prog)
+    want = cpu.store()
+    launches = {}
+    for lay in (0, 1):
+        dev = gpu_state(n)
+        dev.set_option("layout", lay)
+        dev.apply(prog)
+        launches[lay] = dev.profile(reset=True)["fused_pass"]["launches"]
+        dev.apply(prog)
+        got = dev.store()
+        assert np.abs(got - want).max() < 1e-12
+        idx = np.arange(1 << n)
+        p = np.abs(want) ** 2
+        z = dev.expect([("Z", [3]), ("ZZ", [0, n - 1])])
+        want_z = [np.sum(p * (1 - 2 * ((idx >> 3) & 1))), np.sum(p * (1 - 2 * ((idx ^ (idx >> (n - 1))) & 1)))]
+        assert np.allclose(z, want_z, atol=1e-12)
+        dev.close()
+    plans = {lay: _native.plan_json(prog, n, "c128", layout=bool(lay)) for lay in (0, 1)}
+    for lay in (0, 1):
+        assert launches[lay] == sum(1 for it in plans[lay]["items"] if it["fused"])
+    assert launches[1] < launches[0], launches
 
 
 def test_option_validation():

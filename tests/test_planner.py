@@ -17,9 +17,23 @@ F_DENSE, F_REAL, F_DIAG, F_SWAP, F_PHASE, F_FLUSH, F_GRAD, F_RELABEL = range(8)
 G_MULTI = 4
 
 
+def move_bits(psi, n, Q, out_bit):
+    """A pass's store with its tile qubits permuted: the bit at position Q[k] of every index moves to
+    position Q[out_bit[k]] (the amplitudes stay at the tile's addresses)."""
+    idx = np.arange(1 << n, dtype=np.int64)
+    qmask = sum(1 << q for q in Q)
+    dst = idx & ~qmask
+    for k, q in enumerate(Q):
+        dst |= ((idx >> q) & 1) << Q[out_bit[k]]
+    new = np.empty_like(psi)
+    new[dst] = psi
+    return new
+
+
 def replay(plan, prog, n, psi):
-    """Executes a plan's fused ops as controlled one-qubit gates on global qubits, in plan order.
-    Non-fused items are the original MULTI ops, taken from the program in order."""
+    """Executes a plan's fused ops as controlled one-qubit gates on qubit positions, in plan order,
+    and each pass's tile permutation (layout-aware plans). Non-fused items are the program's MULTI
+    ops in order, on the positions the plan gives them."""
     psi = np.array(psi, dtype=complex)
     multi_rows = [r for r in prog.ops if r["code"] == 1]
     mi = 0
@@ -30,9 +44,10 @@ def replay(plan, prog, n, psi):
             mi += 1
             q = prog.idx[row["q_off"]: row["q_off"] + row["n_qubits"]].tolist()
             nc = int(row["n_ctrl"])
+            assert len(it["ctrls"]) == nc and len(it["targets"]) == len(q) - nc
             d = 1 << (len(q) - nc)
             a = prog.mat[row["m_off"]: row["m_off"] + 2 * d * d]
-            psi = dense.apply_unitary(psi, n, q[:nc], q[nc:], (a[0::2] + 1j * a[1::2]).reshape(d, d))
+            psi = dense.apply_unitary(psi, n, it["ctrls"], it["targets"], (a[0::2] + 1j * a[1::2]).reshape(d, d))
             continue
         Q = it["Q"]
         assert Q == sorted(Q) and len(set(Q)) == len(Q)
@@ -73,6 +88,9 @@ def replay(plan, prog, n, psi):
                         m = [z.real for z in m]
                     psi[i0] = m[0] * a + m[1] * b
                     psi[i1] = m[2] * a + m[3] * b
+        if it["out_bit"]:
+            assert sorted(it["out_bit"]) == list(range(len(Q)))
+            psi = move_bits(psi, n, Q, it["out_bit"])
     assert mi == len(multi_rows)
     return psi
 
@@ -95,6 +113,31 @@ def test_random_programs_replay(n, tile, low, groups):
     rng = np.random.Generator(np.random.PCG64(500 + n))
     prog = random_gate_program(n, 300, rng, max_multi=3)
     check(prog, n, tile_bits=tile, low_bits=low, max_groups=groups)
+
+
+@pytest.mark.parametrize("n,tile,low,groups", [(11, 10, 4, 0), (12, 11, 5, 0), (14, 12, 6, 24), (16, 10, 3, 4),
+                                                (17, 13, 5, 6), (18, 11, 5, 12)])
+def test_layout_plans_replay(n, tile, low, groups):
+    """Layout-aware plans (option "layout"): gates run on moving positions, tiles are stored with
+    their qubits permuted, and the plan ends in the standard order (the replay is compared with the
+    oracle's state index by index)."""
+    rng = np.random.Generator(np.random.PCG64(700 + n))
+    prog = random_gate_program(n, 300, rng, max_multi=3)
+    plan = check(prog, n, tile_bits=tile, low_bits=low, max_groups=groups, layout=True)
+    assert any(it.get("out_bit") for it in plan["items"] if it["fused"])
+
+
+def test_layout_plan_brickwork_fewer_passes():
+    n = 20
+    prog = compile_circuit(workloads.random_u3_cx(n, 8, 6, seed=11, measure=False))
+    plain = _native.plan_json(prog, n, "c128")
+    moved = check(prog, n, layout=True)
+    p0 = sum(1 for it in plain["items"] if it["fused"])
+    p1 = sum(1 for it in moved["items"] if it["fused"])
+    assert p1 < 0.85 * p0, (p0, p1)
+    # every gate runs exactly once; the passes at the end carry no gates, only the restoring moves
+    ops = sum(1 for it in moved["items"] if it["fused"] for g in it["groups"] for op in g["ops"] if op["kind"] != F_FLUSH)
+    assert ops == prog.n_ops
 
 
 def test_brickwork_plan_is_compact():
