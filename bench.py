@@ -151,7 +151,16 @@ def copy_calibration() -> int:
 def stream_reference(nbytes: int) -> dict:
     """A plain streaming kernel over a buffer the size of the state: torch's in-place neg_ on
     float64 (reads and writes every byte once), timed with events on torch's current stream, best of
-    3. Its rate is the practical HBM ceiling a pass is compared with."""
+    3, in a child process (torch's own HIP runtime does not share a process with the engine's). Its
+    rate is the practical HBM ceiling a pass is compared with."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-reference", str(nbytes)],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"stream reference failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def stream_reference_child(nbytes: int) -> int:
     import torch
 
     x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
@@ -166,10 +175,10 @@ def stream_reference(nbytes: int) -> dict:
         b.synchronize()
         best = min(best, a.elapsed_time(b))
     del x
-    torch.cuda.empty_cache()
     gbps = 2 * nbytes / (best / 1e3) / 1e9
-    return {"op": "torch neg_ in place, float64", "bytes": 2 * nbytes, "ms": round(best, 4), "GBps": round(gbps, 2),
-            "frac": round(gbps / PEAK_GBPS, 4)}
+    print(json.dumps({"op": "torch neg_ in place, float64", "bytes": 2 * nbytes, "ms": round(best, 4),
+                      "GBps": round(gbps, 2), "frac": round(gbps / PEAK_GBPS, 4)}), flush=True)
+    return 0
 
 
 def adjoint_leg(n: int, layers: int, fd_params: int = 8):
@@ -392,18 +401,18 @@ def batch_leg(count: int, workers: int, graphs: bool = True, steps: int = 3, pin
 
 
 def dense_leg(n: int = 28, reps: int = 3):
-    """Dense gates on k = 4, 5 and 6 targets (seeded Haar-random unitaries, PCG64 seed 20250006;
+    """Dense gates on k = 4 to 8 targets (seeded Haar-random unitaries, PCG64 seed 20250006;
     targets the k lowest, the k highest, or qubits 8 .. 8+k-1) on an n-qubit state, complex128 and
     complex64, on the matrix cores (option mfma = 1: FP64 / FP32 MFMA) and on the vector-ALU kernels
-    (mfma = 0). Per gate: mean HIP-event duration, algorithmic bytes 2 * s * 2^n and flops
-    8 * 2^k * 2^n."""
+    (mfma = 0; for k = 7 and 8 only with the middle targets, one timed call). Per gate: mean
+    HIP-event duration, algorithmic bytes 2 * s * 2^n and flops 8 * 2^k * 2^n."""
     from pyquafu_amd import _native
     from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
 
     rng = np.random.Generator(np.random.PCG64(20250006))
     out = {"workload": f"{n}q, one dense k-target gate", "rows": []}
     mats = {}
-    for k in (4, 5, 6):
+    for k in (4, 5, 6, 7, 8):
         d = 1 << k
         a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
         q, r = np.linalg.qr(a)
@@ -411,19 +420,21 @@ def dense_leg(n: int = 28, reps: int = 3):
     for dtype, amp in (("c128", 16), ("c64", 8)):
         dev = _native.DeviceState(n, dtype)
         dev.set_profiling(True)
-        for k in (4, 5, 6):
+        for k in (4, 5, 6, 7, 8):
             d = 1 << k
             for where, tg in (("low", list(range(k))), ("mid", list(range(8, 8 + k))), ("high", list(range(n - k, n)))):
                 b = ProgramBuilder()
                 b.add(MULTI, tg, matrix=mats[k])
                 prog = b.program(n_qubits=n)
                 for mfma in (1, 0):
+                    if not mfma and k >= 7 and where != "mid":
+                        continue
                     dev.set_option("mfma", mfma)
                     dev.init_zero()
                     dev.apply(prog)
                     dev.synchronize()
                     dev.profile(reset=True)
-                    for _ in range(reps):
+                    for _ in range(reps if mfma or k < 7 else 1):
                         dev.apply(prog)
                     dev.synchronize()
                     prof = dev.profile(reset=True)
@@ -452,6 +463,7 @@ def main(argv=None) -> int:
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--max-groups", type=int, default=0)
+    ap.add_argument("--stream-reference", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the streaming calibration kernel for the HBM counters")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-parameter-shift leg")
@@ -484,6 +496,8 @@ def main(argv=None) -> int:
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.copy_calibration:
         return copy_calibration()
+    if args.stream_reference:
+        return stream_reference_child(args.stream_reference)
 
     from pyquafu_amd import _native, workloads
     from pyquafu_amd.simulators.program import compile_circuit
@@ -628,7 +642,7 @@ def main(argv=None) -> int:
     else:
         out["precision"] = None
     if not args.no_dense:
-        print("[bench] dense gate leg: k = 4, 5, 6 on 28 qubits", file=sys.stderr, flush=True)
+        print("[bench] dense gate leg: k = 4 to 8 on 28 qubits", file=sys.stderr, flush=True)
         out["dense"] = dense_leg()
     else:
         out["dense"] = None

@@ -330,7 +330,7 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 0 = one workgroup per tile
  *   "stagger"     persistent kernel: start delay of half the workgroups, in 8128-cycle sleeps (0..64)
  *   "stagger_shift" the blockIdx bit that selects the delayed half (0..16)
- *   "mfma"        1 = (default) dense gates on 4 to 6 targets run on the matrix cores (FP64 MFMA at
+ *   "mfma"        1 = (default) dense gates on 4 to 8 targets run on the matrix cores (FP64 MFMA at
  *                 complex128, FP32 MFMA at complex64), 0 = on the vector ALUs
  *   "checkpoints" K (0..64, default 0): qfvm_adjoint keeps up to K copies of the forward state at
  *                 op boundaries spread over the differentiated range and restores psi from them in

@@ -155,7 +155,7 @@ struct State {
     // adjoint checkpoints (option "checkpoints"): up to this many copies of the forward state,
     // restored into psi during the reverse sweep instead of continuing the uncompute past them
     int checkpoints = 0;
-    int mfma = 1;  // dense gates on 4 to 6 targets run on the matrix cores (option "mfma")
+    int mfma = 1;  // dense gates on 4 to 8 targets run on the matrix cores (option "mfma")
     std::vector<void*> ckpt;
     // graph mode (option "graphs"): qfvm_adjoint captures its stream work into a hipGraph and
     // replays it through an executable graph cached per program structure (see capi.cpp)

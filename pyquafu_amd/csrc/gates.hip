@@ -255,11 +255,125 @@ __global__ void __launch_bounds__(256, K == 6 ? 2 : 4) k_multi_mfma(V2<R>* __res
     }
 }
 
-// Dense k-target matrix, K >= 6: one block per group; the group is staged in LDS.
 struct TPos {
     int k;
     int t[16];
 };
+
+// Dense gates on K = 7 or 8 targets on the matrix cores. The real embedding A (M x M, M = 2^(K+1):
+// 512 KiB / 2 MiB at complex128) no longer fits in registers or LDS, so it streams from memory (L2
+// holds it: every workgroup reads the same matrix) while the workgroup's tile X of NG groups sits
+// in LDS. Wavefront w owns the 16-row blocks w, w + 4, .. of Y = A X and all NG / 16 column blocks;
+// per k step it loads its A fragments for the next step (one step ahead) and takes X from LDS.
+// NG is chosen so a tile is ~68 KiB: two workgroups per CU.
+template <typename R, int K>
+constexpr int wide_ng() { return (sizeof(R) == 8 ? 32 : 64) >> (K - 7); }
+template <typename R, int K>
+constexpr size_t wide_lds() {
+    return (size_t)(2 << K) * (wide_ng<R, K>() + 1) * sizeof(R) + (wide_ng<R, K>() + (1 << K)) * sizeof(uint64_t);
+}
+
+template <typename R, int K>
+__global__ void __launch_bounds__(256, 2) k_multi_wide(V2<R>* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
+                                                       uint64_t cmask, TPos tp, const R* __restrict__ at,
+                                                       int member_fast, uint64_t tiles) {
+    typedef typename MfmaOp<R>::v4 v4;
+    constexpr int D = 1 << K, M = 2 * D, NG = wide_ng<R, K>(), P = NG + 1, KS = M / 4;
+    constexpr int NR = M / 64, NC = NG / 16;  // row blocks per wavefront, column blocks
+    constexpr int ITER = D * NG / 256;         // elements per thread and tile
+    extern __shared__ unsigned char lds_raw[];
+    R* const xs = reinterpret_cast<R*>(lds_raw);
+    uint64_t* const gbase = reinterpret_cast<uint64_t*>(lds_raw + (size_t)M * P * sizeof(R));
+    uint64_t* const moff = gbase + NG;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, l16 = lane & 15;
+    auto where = [&](int e, int& m, int& c) {
+        if (member_fast) {
+            m = e & (D - 1);
+            c = e >> K;
+        } else {
+            c = e & (NG - 1);
+            m = e / NG;
+        }
+    };
+    for (int m = tid; m < D; m += 256) {
+        uint64_t o = 0;
+        for (int b = 0; b < K; b++) o |= (uint64_t)((m >> b) & 1) << tp.t[b];
+        moff[m] = o;
+    }
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t g0 = t * NG;
+        if (tid < NG) {
+            const uint64_t g = g0 + (uint64_t)tid;
+            gbase[tid] = g < ngroups ? (insert_zeros(g, sp) | cmask) : ~0ull;
+        }
+        __syncthreads();
+        constexpr int CH = 8;
+#pragma unroll
+        for (int i0 = 0; i0 < ITER; i0 += CH) {
+            V2<R> v[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int m, c;
+                where(tid + 256 * (i0 + i), m, c);
+                const uint64_t gb = gbase[c];
+                v[i] = gb != ~0ull ? psi[gb | moff[m]] : mk<R>(0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int m, c;
+                where(tid + 256 * (i0 + i), m, c);
+                xs[m * P + c] = v[i].x;
+                xs[(m + D) * P + c] = v[i].y;
+            }
+        }
+        __syncthreads();
+        v4 acc[NR][NC];
+#pragma unroll
+        for (int j = 0; j < NR; j++)
+#pragma unroll
+            for (int ct = 0; ct < NC; ct++) acc[j][ct] = v4{0, 0, 0, 0};
+        // A[(wave + 4 j) 16 + l16][4 ks + kq] = at[(4 ks + kq) M + (wave + 4 j) 16 + l16]
+        const R* ap = at + (size_t)kq * M + wave * 16 + l16;
+        R a[NR], an[NR];
+#pragma unroll
+        for (int j = 0; j < NR; j++) a[j] = ap[j * 64];
+#pragma unroll 2
+        for (int ks = 0; ks < KS; ks++) {
+            if (ks + 1 < KS) {
+#pragma unroll
+                for (int j = 0; j < NR; j++) an[j] = ap[(size_t)(ks + 1) * 4 * M + j * 64];
+            }
+            R b[NC];
+#pragma unroll
+            for (int ct = 0; ct < NC; ct++) b[ct] = xs[(4 * ks + kq) * P + ct * 16 + l16];
+#pragma unroll
+            for (int j = 0; j < NR; j++)
+#pragma unroll
+                for (int ct = 0; ct < NC; ct++) acc[j][ct] = MfmaOp<R>::mma(a[j], b[ct], acc[j][ct]);
+#pragma unroll
+            for (int j = 0; j < NR; j++) a[j] = an[j];
+        }
+        __syncthreads();  // every wavefront has read X
+#pragma unroll
+        for (int j = 0; j < NR; j++)
+#pragma unroll
+            for (int ct = 0; ct < NC; ct++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    xs[((wave + 4 * j) * 16 + MfmaOp<R>::row(v, kq)) * P + ct * 16 + l16] = acc[j][ct][v];
+        __syncthreads();
+#pragma unroll 4
+        for (int i = 0; i < ITER; i++) {
+            int m, c;
+            where(tid + 256 * i, m, c);
+            const uint64_t gb = gbase[c];
+            if (gb != ~0ull) psi[gb | moff[m]] = mk<R>(xs[m * P + c], xs[(m + D) * P + c]);
+        }
+        __syncthreads();  // the next tile rewrites xs and gbase
+    }
+}
+
+// Dense k-target matrix, K >= 6: one block per group; the group is staged in LDS.
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_multi_lds(V2<R>* __restrict__ psi, uint64_t ngroups, InsertSpec sp,
@@ -405,18 +519,12 @@ static void multi_reg_launch(State& s, const Gate& g, const InsertSpec& sp, uint
     QFVM_LAUNCHED();
 }
 
-template <typename R, int K>
-static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uint64_t ngroups) {
-    constexpr int D = 1 << K, M = 2 * D;
-    TMask<K> tm{};
-    for (int m = 0; m < D; m++) {
-        uint64_t o = 0;
-        for (int b = 0; b < K; b++)
-            if ((m >> b) & 1) o |= 1ull << g.targets[b];
-        tm.m[m] = o;
-    }
-    // A^T of the real embedding, at[k * M + r] = A[r][k], in the state's precision, staged through scratch
-    std::vector<R> at((size_t)M * M);
+// A^T of the real embedding of the gate's matrix, at[k * M + r] = A[r][k] (A = [[Ur, -Ui], [Ui, Ur]]),
+// in the state's precision
+template <typename R>
+static std::vector<R> real_embedding_t(const Gate& g, int D) {
+    const size_t M = 2 * (size_t)D;
+    std::vector<R> at(M * M);
     for (int r = 0; r < D; r++)
         for (int q = 0; q < D; q++) {
             const cd u = g.mk[(size_t)r * D + q];
@@ -425,14 +533,59 @@ static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uin
             at[(size_t)q * M + r + D] = (R)u.imag();
             at[(size_t)(q + D) * M + r + D] = (R)u.real();
         }
-    R* dat = reinterpret_cast<R*>(s.ensure_scratch(at.size() * sizeof(R)));
-    QFVM_HIP(hipMemcpyAsync(dat, at.data(), at.size() * sizeof(R), hipMemcpyHostToDevice, s.stream));
+    return at;
+}
+
+// gather order of the matrix-core kernels: member-fast when a target is below every other free
+// qubit (consecutive members are then the closest addresses), column-fast otherwise
+static int member_fast_order(const State& s, const Gate& g) {
     int lowest = 64;
     for (int t : g.targets) lowest = std::min(lowest, t);
     int lowest_free = 0;  // lowest qubit outside the gate
     while (lowest_free < s.n && (((g.cmask >> lowest_free) & 1) || std::find(g.targets.begin(), g.targets.end(), lowest_free) != g.targets.end()))
         lowest_free++;
-    const int member_fast = lowest < lowest_free ? 1 : 0;
+    return lowest < lowest_free ? 1 : 0;
+}
+
+template <typename R, int K>
+static void multi_wide_launch(State& s, const Gate& g, const InsertSpec& sp, uint64_t ngroups) {
+    constexpr int D = 1 << K;
+    TPos tp{};
+    tp.k = K;
+    for (int b = 0; b < K; b++) tp.t[b] = g.targets[b];
+    const std::vector<R> at = real_embedding_t<R>(g, D);
+    R* dat = reinterpret_cast<R*>(s.ensure_scratch(at.size() * sizeof(R)));
+    QFVM_HIP(hipMemcpyAsync(dat, at.data(), at.size() * sizeof(R), hipMemcpyHostToDevice, s.stream));
+    const size_t lds = wide_lds<R, K>();
+    const void* fn = reinterpret_cast<const void*>(&k_multi_wide<R, K>);
+    if (lds > 65536) allow_large_lds(fn, s.device);
+    const uint64_t tiles = (ngroups + wide_ng<R, K>() - 1) / wide_ng<R, K>();
+    int cus = 0, per_cu = 0;
+    QFVM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device));
+    QFVM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1));
+    KernelScope ks(s, "multi_mfma", (double)ngroups * D * 2.0 * sizeof(V2<R>));
+    hipLaunchKernelGGL((k_multi_wide<R, K>), dim3(grid), dim3(256), lds, s.stream, reinterpret_cast<V2<R>*>(s.psi),
+                       ngroups, sp, g.cmask, tp, dat, member_fast_order(s, g), tiles);
+    QFVM_LAUNCHED();
+    QFVM_HIP(hipStreamSynchronize(s.stream));  // the host matrix copy must outlive the async copy
+}
+
+template <typename R, int K>
+static void multi_mfma_launch(State& s, const Gate& g, const InsertSpec& sp, uint64_t ngroups) {
+    constexpr int D = 1 << K;
+    TMask<K> tm{};
+    for (int m = 0; m < D; m++) {
+        uint64_t o = 0;
+        for (int b = 0; b < K; b++)
+            if ((m >> b) & 1) o |= 1ull << g.targets[b];
+        tm.m[m] = o;
+    }
+    // A^T of the real embedding, staged through scratch
+    const std::vector<R> at = real_embedding_t<R>(g, D);
+    R* dat = reinterpret_cast<R*>(s.ensure_scratch(at.size() * sizeof(R)));
+    QFVM_HIP(hipMemcpyAsync(dat, at.data(), at.size() * sizeof(R), hipMemcpyHostToDevice, s.stream));
+    const int member_fast = member_fast_order(s, g);
     const size_t lds = mfma_lds<R, K>();
     const void* fn = reinterpret_cast<const void*>(&k_multi_mfma<R, K>);
     if (lds > 65536) allow_large_lds(fn, s.device);
@@ -458,13 +611,14 @@ static void multi_typed(State& s, const Gate& g) {
     pos.insert(pos.end(), g.targets.begin(), g.targets.end());
     const InsertSpec sp = make_spec(pos);
     const uint64_t ngroups = s.size >> pos.size();
-    if (s.mfma && k >= 4 && k <= 6) {
-        if (k == 4)
-            multi_mfma_launch<R, 4>(s, g, sp, ngroups);
-        else if (k == 5)
-            multi_mfma_launch<R, 5>(s, g, sp, ngroups);
-        else
-            multi_mfma_launch<R, 6>(s, g, sp, ngroups);
+    if (s.mfma && k >= 4 && k <= 8) {
+        switch (k) {
+            case 4: multi_mfma_launch<R, 4>(s, g, sp, ngroups); break;
+            case 5: multi_mfma_launch<R, 5>(s, g, sp, ngroups); break;
+            case 6: multi_mfma_launch<R, 6>(s, g, sp, ngroups); break;
+            case 7: multi_wide_launch<R, 7>(s, g, sp, ngroups); break;
+            default: multi_wide_launch<R, 8>(s, g, sp, ngroups); break;
+        }
         return;
     }
     // matrix -> device (scratch), converted to the state's precision

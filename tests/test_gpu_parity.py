@@ -647,12 +647,15 @@ def test_full_size_round_trip_30_qubits():
 # ------------------------------------------------------------------ dense gates on the matrix cores
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("k,where,nc", [(4, "low", 0), (4, "mixed", 1), (5, "low", 0), (5, "high", 1),
-                                        (5, "mixed", 2), (6, "low", 1), (6, "high", 0), (6, "mixed", 2)])
+                                        (5, "mixed", 2), (6, "low", 1), (6, "high", 0), (6, "mixed", 2),
+                                        (7, "low", 0), (7, "high", 1), (7, "mixed", 2), (8, "low", 0),
+                                        (8, "mixed", 1), (8, "high", 3)])
 def test_dense_mfma_matches_oracle(k, where, nc, dtype):
-    """Dense gates on 4 to 6 targets run on the matrix cores (option mfma, the default:
-    v_mfma_f64_16x16x4f64 at complex128, v_mfma_f32_16x16x4f32 at complex64); against the oracle
-    and against the VALU kernels (mfma = 0), with the targets among the lowest qubits (member-fast
-    tile order), the highest, or spread, and with controls."""
+    """Dense gates on 4 to 8 targets run on the matrix cores (option mfma, the default:
+    v_mfma_f64_16x16x4f64 at complex128, v_mfma_f32_16x16x4f32 at complex64; 7 and 8 targets with the
+    matrix streamed from memory); against the oracle and against the VALU kernels (mfma = 0), with
+    the targets among the lowest qubits (member-fast tile order), the highest, or spread, and with
+    controls (8 targets and 3 controls at 15 qubits leave 16 groups: a partly filled tile)."""
     from pyquafu_amd.simulators.program import MULTI, ProgramBuilder
     from tests.progs import random_unitary
 
