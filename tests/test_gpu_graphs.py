@@ -232,3 +232,33 @@ def test_value_dependent_plan_gets_its_own_executable():
     finally:
         eager.close()
         graph.close()
+
+
+def test_layout_option_in_adjoint_eager_and_graphs():
+    """Option "layout" also plans the adjoint's forward passes (each call ends in the standard qubit
+    order): eager against the default planner and the oracle, and graph replays against eager calls
+    with the same option (same kernels in the same order: equal bits)."""
+    n = 15
+    ham = tfim(n)
+    plain = _native.DeviceState(n, "c128")
+    eager = _native.DeviceState(n, "c128")
+    graph = _native.DeviceState(n, "c128")
+    try:
+        eager.set_option("layout", 1)
+        graph.set_option("layout", 1)
+        graph.set_option("graphs", 1)
+        for rep in range(3):
+            args = call_args(layered(n, 4, 900 + rep), ham)
+            g0, e0 = run(plain, args)
+            g1, e1 = run(eager, args)
+            g2, e2 = run(graph, args)
+            assert np.abs(g1 - g0).max() < 1e-12 and abs(e1 - e0) < 1e-12
+            assert np.array_equal(g2, g1) and e2 == e1
+            want, e_want = og.adjoint(args[0], n, *args[1:])
+            assert abs(e1 - e_want) < 1e-11
+            assert np.abs(g1 - want).max() < 1e-10
+        assert graph_counts(graph)["graph_launch"] >= 2
+    finally:
+        plain.close()
+        eager.close()
+        graph.close()
