@@ -176,3 +176,18 @@ def test_plan_depends_on_parameter_values():
     p0 = _native.plan_json(prog(0.0), 14, "c128")
     p1 = _native.plan_json(prog(0.3), 14, "c128")
     assert p0 != p1
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_layout_plans_replay_random_shapes(seed):
+    """Layout-aware plans over random sizes and tile shapes (tile 10..13 bits, 2..6 low bits, 1..24
+    groups), random programs with controlled and multi-target gates, replayed against the oracle."""
+    rng = np.random.Generator(np.random.PCG64(9000 + seed))
+    tile = int(rng.integers(10, 14))
+    n = int(rng.integers(tile + 1, 17))
+    low = int(rng.integers(2, 7))
+    groups = int(rng.integers(1, 25))
+    prog = random_gate_program(n, 200, rng, max_multi=3)
+    plan = check(prog, n, tile_bits=tile, low_bits=low, max_groups=groups, layout=True)
+    fused = [it for it in plan["items"] if it["fused"]]
+    assert all(len(it["Q"]) == tile for it in fused)
