@@ -49,12 +49,12 @@ def random_gate_program(n: int, n_ops: int, rng, max_multi: int = 3, max_ctrl: i
     return b.program(n_qubits=n, n_cbits=n)
 
 
-def random_classical_program(n: int, rng, n_blocks: int = 4):
+def random_classical_program(n: int, rng, n_blocks: int = 4, block_gates: int = 6):
     """Gates interleaved with measure / reset / cif blocks."""
     b = ProgramBuilder()
     n_random = 0
     for _ in range(n_blocks):
-        sub = random_gate_program(n, 6, rng)
+        sub = random_gate_program(n, block_gates, rng)
         for row in sub.ops:
             q = sub.idx[row["q_off"]: row["q_off"] + row["n_qubits"]]
             m = None

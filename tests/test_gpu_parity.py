@@ -212,6 +212,28 @@ def test_classical_programs(seed, dtype):
     assert np.abs(dev.store() - cpu.store()).max() < (1e-12 if dtype == "c128" else 1e-5)
 
 
+@pytest.mark.parametrize("seed", [61, 62])
+def test_layout_with_mid_circuit_operations(seed):
+    """Option "layout" (with 2^10-amplitude tiles on 13 qubits) on gate runs of 150 gates between
+    measurements, resets and classically controlled gates: every gate run ends in the standard qubit
+    order, so the collapses, the classical register and the final state match the oracle."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 13
+    prog = random_classical_program(n, rng, n_blocks=4, block_gates=150)
+    u = rng.random(prog.n_random)
+    psi0 = rand_state(n, rng)
+    cpu = CpuState(n)
+    cpu.load(psi0)
+    want_creg = cpu.run(prog, u, n)
+    dev = gpu_state(n)
+    dev.set_option("tile_bits", 10)
+    dev.set_option("layout", 1)
+    dev.load(psi0)
+    got_creg = dev.run(prog, u, n)
+    assert np.array_equal(got_creg, want_creg)
+    assert np.abs(dev.store() - cpu.store()).max() < 1e-12
+
+
 def test_parity_workload_20q():
     """20 qubits, complex128, 20 layers of U3 on every qubit + CX brickwork, 10^5 shots."""
     n = 20
