@@ -77,6 +77,49 @@ def test_dense_identities(rng):
     assert np.allclose(f, np.eye(8)[0b100])
 
 
+def _full_matrix(n, ctrls, targs, U):
+    """The whole 2^n x 2^n operator of a controlled k-target gate, built entry by entry from basis
+    states: column i maps to rows with the target bits replaced (U[r', r], bit j of r <-> targs[j])
+    where every control bit of i is set, and to itself elsewhere."""
+    N = 1 << n
+    M = np.zeros((N, N), dtype=complex)
+    for i in range(N):
+        if not all((i >> c) & 1 for c in ctrls):
+            M[i, i] = 1.0
+            continue
+        r = sum(((i >> t) & 1) << j for j, t in enumerate(targs))
+        base = i & ~sum(1 << t for t in targs)
+        for r2 in range(1 << len(targs)):
+            row = base | sum(((r2 >> j) & 1) << t for j, t in enumerate(targs))
+            M[row, i] += U[r2, r]
+    return M
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_multi_target_dense_against_full_matrix(seed):
+    """Brute force on 5 qubits: the oracle's controlled k-target gate (k = 1..4, 0..2 controls, any
+    target order) equals the explicit 2^n x 2^n operator applied to the state, and a Kronecker
+    product gate equals its factors applied one target at a time."""
+    rng = np.random.Generator(np.random.PCG64(4200 + seed))
+    n = 5
+    k = int(rng.integers(1, 5))
+    nc = int(rng.integers(0, min(2, n - k) + 1))
+    qs = [int(x) for x in rng.permutation(n)]
+    targs, ctrls = qs[:k], qs[k:k + nc]
+    a = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+    U, _ = np.linalg.qr(a)
+    psi = rand_state(n, rng)
+    got = dense.apply_unitary(psi, n, ctrls, targs, U)
+    assert np.allclose(got, _full_matrix(n, ctrls, targs, U) @ psi, atol=1e-13)
+    if k >= 2:
+        A, _ = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))
+        B, _ = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))
+        # row bit 0 <-> targs[0]: kron(A, B) is B on targs[0] and A on targs[1]
+        both = dense.apply_unitary(psi, n, ctrls, targs[:2], np.kron(A, B))
+        step = dense.apply_unitary(dense.apply_unitary(psi, n, ctrls, [targs[0]], B), n, ctrls, [targs[1]], A)
+        assert np.allclose(both, step, atol=1e-13)
+
+
 @pytest.mark.parametrize("n", [1, 3, 6, 9])
 def test_qft_closed_form_against_fft(n):
     x = (0b1011011 * 7919) % (1 << n)
