@@ -350,7 +350,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
  * The execution plan qfvm_apply would run for `prog` on an n-qubit state, as JSON (host only, no
  * device needed; no counterpart in the reference). Fused passes list their tile qubits (positions),
  * "out_bit" (layout-aware plans: where each tile bit is stored; empty = in place) and, per group,
- * the ops in execution order: {"kind", "g", "cmg", "cme", "run", "m": [re, im] x 4}; other items
+ * the ops in execution order: {"kind", "g", "g2", "cmg", "cme", "run", "m": [re, im] x 4, or x 16
+ * for a 4x4 on group bits g and g2 (kind 9)}; other items
  * carry the gate on the positions it runs at. Lets the planner's reordering be checked without a
  * GPU. Arguments of 0 pick the defaults, as for qfvm_set_option; layout = 1 plans as option
  * "layout" does.

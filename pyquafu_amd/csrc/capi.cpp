@@ -1213,12 +1213,15 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
                     for (size_t o = 0; o < G.ops.size(); o++) {
                         const FOpHost& op = G.ops[o];
                         std::snprintf(t, sizeof(t),
-                                      "%s{\"kind\": %d, \"g\": %d, \"cmg\": %u, \"cme\": %llu, \"run\": %d, \"flags\": %u, "
-                                      "\"m\": [",
-                                      o ? ", " : "", op.kind, op.g, op.cm_group, (unsigned long long)op.cm_ext, op.run,
-                                      op.flags);
+                                      "%s{\"kind\": %d, \"g\": %d, \"g2\": %d, \"cmg\": %u, \"cme\": %llu, \"run\": %d, "
+                                      "\"flags\": %u, \"m\": [",
+                                      o ? ", " : "", op.kind, op.g, op.kind == F_DENSE4 ? op.target : -1, op.cm_group,
+                                      (unsigned long long)op.cm_ext, op.run, op.flags);
                         j += t;
-                        for (int k = 0; k < 4; k++) j += (k ? ", " : "") + cplx(op.m[k]);
+                        if (op.kind == F_DENSE4)  // the 4x4, row-major
+                            for (size_t k = 0; k < op.mx.size(); k++) j += (k ? ", " : "") + cplx(op.mx[k]);
+                        else
+                            for (int k = 0; k < 4; k++) j += (k ? ", " : "") + cplx(op.m[k]);
                         j += "]}";
                     }
                     j += "]}";

@@ -24,7 +24,9 @@ enum FKind : int32_t {
     F_FLUSH = 5,  // apply the pending relabeling of group bit g to the data (mask in pxmask)
     F_GRAD = 6,   // gradient marker (stage groups only): g = GradGen, see FOpHost
     F_RELABEL = 7, // host only: an X folded into the relabeling of bit g (listed, never uploaded)
-    F_DIAGRUN = 8  // device only: a run of diagonal ops applied at once (see below)
+    F_DIAGRUN = 8,  // device only: a run of diagonal ops applied at once (see below)
+    F_DENSE4 = 9    // dense 4x4 on group bits g (row/column bit 0) and target (bit 1), matrix in mx
+                    //   (row-of-8 groups only)
 };
 
 // Diagonal runs. Diagonal gates commute, so consecutive diagonal ops of a group (relabelings may
@@ -65,6 +67,7 @@ struct FOpHost {
     uint64_t pxmask;    // FL_TARGET_PX / F_FLUSH: relabel parity mask of the op's bit g
     int32_t run;        // 1 + index of the diagonal run the op belongs to in its group (0: none)
     int32_t src;        // index of the gate the op came from in the planned list (-1: none)
+    std::vector<cd> mx; // F_DENSE4: the 4x4 matrix, row-major (index bit 0 <-> group bit g)
 };
 
 // A group is either a compute group (4 tile qubits and their gates) or, in the two-state reverse

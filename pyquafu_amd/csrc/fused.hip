@@ -54,6 +54,7 @@ struct PassDesc {
     // 64 elements per store, chosen so the LDS reads avoid bank conflicts), read from slot
     // swz(XOR of pbit[k] = Pinv(rbit[k])) ^ pst[j]; permuted = 0 stores every element in place
     uint32_t permuted;
+    uint32_t has_d4;  // the pass holds 4x4 ops (host side: picks the kernel instance)
     int32_t pbit[FUSE_MAX_TILE_BITS];
     int32_t rbit[FUSE_MAX_TILE_BITS];
     int32_t pst[8];
@@ -1155,8 +1156,11 @@ enum K8Kind : int32_t {
                 //   group, with group / outside controls and relabel swaps
     K8_FL = 4,  // relabel flush of group bit g (pairs swap where the lane's parity is set)
     K8_DR = 5,  // diagonal run (the split-row kernel's records, subsets of 3 bits)
-    K8_SW = 6   // exchange of the pair members on group bit g where the group controls are set
+    K8_SW = 6,  // exchange of the pair members on group bit g where the group controls are set
                 //   (X and controlled X inside the group): register moves only
+    K8_D4 = 7   // dense 4x4 on group bits g & 15 (index bit 0) and g >> 4 (bit 1), the matrix (16
+                //   complex, row-major) at byte offset pxmask of the run buffer; a group control on
+                //   the third bit (cm) and controls outside the group (cm_ext) as for K8_G2
 };
 
 // An op record as the row-of-8 kernel reads it (wide scalar loads of the header, the outside-
@@ -1334,7 +1338,64 @@ __device__ __forceinline__ void r8_run(const V2<R> (&a)[8], V2<R> (&o)[8], const
      ...);
 }
 
-template <typename R>
+// A 4x4 on group bits T0 (index bit 0) and T1 (bit 1) of the row, for both values of the third bit
+// (CM = 1: a group control on the third bit, so only its set half changes). Where the lane's outside
+// controls do not hold (ok false), the members are copied. The matrix is read one row at a time
+// (8 scalar registers of coefficients live at once; the compiler barrier keeps the loads of the
+// next row from being hoisted).
+template <typename R, int T0, int T1, int CM>
+__device__ __forceinline__ void r8_d4(const V2<R> (&a)[8], V2<R> (&o)[8], const R* __restrict__ M, bool ok) {
+    constexpr int TB = 3 - T0 - T1;
+    constexpr int P0 = 0, P1 = 1 << T0, P2 = 1 << T1, P3 = (1 << T0) | (1 << T1), H = 1 << TB;
+    V2<R> y0[4], y1[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        asm volatile("" ::: "memory");
+        R m[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) m[k] = M[8 * r + k];
+        auto dot = [&](V2<R> x0, V2<R> x1, V2<R> x2, V2<R> x3) {
+            R re = m[0] * x0.x - m[1] * x0.y;
+            R im = m[0] * x0.y + m[1] * x0.x;
+            re = fma(m[2], x1.x, fma(-m[3], x1.y, re));
+            im = fma(m[2], x1.y, fma(m[3], x1.x, im));
+            re = fma(m[4], x2.x, fma(-m[5], x2.y, re));
+            im = fma(m[4], x2.y, fma(m[5], x2.x, im));
+            re = fma(m[6], x3.x, fma(-m[7], x3.y, re));
+            im = fma(m[6], x3.y, fma(m[7], x3.x, im));
+            return mk<R>(re, im);
+        };
+        y1[r] = dot(a[H | P0], a[H | P1], a[H | P2], a[H | P3]);
+        if constexpr (CM == 0) y0[r] = dot(a[P0], a[P1], a[P2], a[P3]);
+    }
+    o[H | P0] = csel<R>(ok, y1[0], a[H | P0]);
+    o[H | P1] = csel<R>(ok, y1[1], a[H | P1]);
+    o[H | P2] = csel<R>(ok, y1[2], a[H | P2]);
+    o[H | P3] = csel<R>(ok, y1[3], a[H | P3]);
+    if constexpr (CM == 0) {
+        o[P0] = csel<R>(ok, y0[0], a[P0]);
+        o[P1] = csel<R>(ok, y0[1], a[P1]);
+        o[P2] = csel<R>(ok, y0[2], a[P2]);
+        o[P3] = csel<R>(ok, y0[3], a[P3]);
+    } else {
+        o[P0] = a[P0];
+        o[P1] = a[P1];
+        o[P2] = a[P2];
+        o[P3] = a[P3];
+    }
+}
+
+template <typename R, int T0, int T1>
+__device__ __forceinline__ void r8_d4_cm(const V2<R> (&a)[8], V2<R> (&o)[8], int cm, const R* __restrict__ M, bool ok) {
+    if (cm == 0)
+        r8_d4<R, T0, T1, 0>(a, o, M, ok);
+    else
+        r8_d4<R, T0, T1, 1>(a, o, M, ok);
+}
+
+// D4: the pass holds 4x4 ops (K8_D4); kernels without them leave the case out, so the common
+// op dispatch keeps its code size and register allocation
+template <typename R, bool D4>
 __device__ __forceinline__ void r8_op(const V2<R> (&a)[8], V2<R> (&o)[8], const Op8<R>& op, uint64_t G0,
                                       const unsigned char* __restrict__ runs) {
     switch (op.kind) {
@@ -1401,6 +1462,23 @@ __device__ __forceinline__ void r8_op(const V2<R> (&a)[8], V2<R> (&o)[8], const 
             }
             break;
         }
+        case K8_D4: {
+            if constexpr (D4) {
+                const R* M = reinterpret_cast<const R*>(runs + op.pxmask);
+                const bool ok = (G0 & op.cm_ext) == op.cm_ext;
+                switch (op.g) {
+                    case 0x10: r8_d4_cm<R, 0, 1>(a, o, op.cm, M, ok); break;
+                    case 0x01: r8_d4_cm<R, 1, 0>(a, o, op.cm, M, ok); break;
+                    case 0x20: r8_d4_cm<R, 0, 2>(a, o, op.cm, M, ok); break;
+                    case 0x02: r8_d4_cm<R, 2, 0>(a, o, op.cm, M, ok); break;
+                    case 0x21: r8_d4_cm<R, 1, 2>(a, o, op.cm, M, ok); break;
+                    default: r8_d4_cm<R, 2, 1>(a, o, op.cm, M, ok); break;
+                }
+                break;
+            }
+            // kernels without 4x4 ops: the label joins the default case (never taken)
+            [[fallthrough]];
+        }
         default:
             r8_run<R>(a, o, runs, op, G0, std::make_integer_sequence<int, 8>{});
             break;
@@ -1439,7 +1517,7 @@ __device__ __forceinline__ void pass8_load(const V2<R>* __restrict__ psi, V2<R>*
 
 // The pass's groups on the tile in LDS (at byte address `tb`): each group gathers its 8 members,
 // runs its ops and scatters them back under its relabeling; every group ends at a barrier.
-template <typename R>
+template <typename R, bool D4>
 __device__ __forceinline__ void pass8_groups(unsigned char* tb, int tid, uint64_t base, const PassDesc& pd,
                                              const FOpDev<R>* __restrict__ ops, const unsigned char* __restrict__ runs) {
     for (int gi = 0; gi < pd.ngroups; gi++) {
@@ -1458,13 +1536,13 @@ __device__ __forceinline__ void pass8_groups(unsigned char* tb, int tid, uint64_
         int oi = pd.gop[gi];
         if (oi < o_end) {
             for (;;) {
-                r8_op<R>(A, B, load_op8<R>(ops + oi), G0, runs);
+                r8_op<R, D4>(A, B, load_op8<R>(ops + oi), G0, runs);
                 if (++oi >= o_end) {
 #pragma unroll
                     for (int m = 0; m < 8; m++) A[m] = B[m];
                     break;
                 }
-                r8_op<R>(B, A, load_op8<R>(ops + oi), G0, runs);
+                r8_op<R, D4>(B, A, load_op8<R>(ops + oi), G0, runs);
                 if (++oi >= o_end) break;
             }
         }
@@ -1487,8 +1565,8 @@ __device__ __forceinline__ void pass8_groups(unsigned char* tb, int tid, uint64_
     }
 }
 
-// One tile per workgroup.
-template <typename R>
+// One tile per workgroup. D4: the pass holds 4x4 ops.
+template <typename R, bool D4>
 __global__ void __launch_bounds__(1024) k_pass8(V2<R>* __restrict__ psi, const PassDesc pd,
                                                 const FOpDev<R>* __restrict__ ops, const unsigned char* __restrict__ runs) {
     extern __shared__ unsigned char lds_raw[];
@@ -1499,7 +1577,7 @@ __global__ void __launch_bounds__(1024) k_pass8(V2<R>* __restrict__ psi, const P
     pass8_load<R>(psi, tile, base, tid, pd, NT, wave);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    pass8_groups<R>(lds_raw, tid, base, pd, ops, runs);
+    pass8_groups<R, D4>(lds_raw, tid, base, pd, ops, runs);
     const uint64_t g_tid = base | local_to_global_low(tid, pd);
     if (pd.permuted) {  // the tile's qubits change places: element e is written from slot swz(Pinv(e))
         int pt = 0, rt = 0;
@@ -1767,10 +1845,29 @@ static PassDesc encode_pass(const State& s, const FPass& p, std::vector<FOpDev<R
                         d.g = -1 - o.g;
                         break;
                     case F_FLUSH: d.kind = K8_FL; break;
+                    case F_DENSE4: {
+                        // the 4x4 goes to the run buffer (16-byte aligned), its offset into pxmask
+                        if (o.mx.size() != 16 || o.g < 0 || o.g > 2 || o.target < 0 || o.target > 2 || o.g == o.target ||
+                            (o.cm_group & ((1u << o.g) | (1u << o.target))))
+                            throw Error(QFVM_ERR_RUNTIME, "malformed 4x4 op");
+                        d.kind = K8_D4;
+                        d.g = o.g | (o.target << 4);
+                        pd.has_d4 = 1;
+                        const size_t off = (runs.size() + 15) / 16 * 16;
+                        runs.resize(off + 32 * sizeof(R));
+                        for (int k = 0; k < 16; k++) {
+                            const R v[2] = {(R)o.mx[(size_t)k].real(), (R)o.mx[(size_t)k].imag()};
+                            std::memcpy(runs.data() + off + (size_t)k * 2 * sizeof(R), v, sizeof(v));
+                        }
+                        d.pxmask = off;
+                        break;
+                    }
                     default: throw Error(QFVM_ERR_RUNTIME, "op kind not supported by the row-of-8 kernel");
                 }
                 // ops that read the lane's global index
                 if (lane || o.kind == F_PHASE || o.kind == F_FLUSH) pd.g0_mask |= 1u << gi;
+            } else if (o.kind == F_DENSE4) {
+                throw Error(QFVM_ERR_RUNTIME, "4x4 ops run in row-of-8 passes only");
             }
             h.push_back(d);
         }
@@ -1934,8 +2031,9 @@ static void execute_typed(State& s, const std::vector<PlanItem>& plan, void* lam
                 const unsigned threads8 = (1u << pd.b) >> 3;
                 KernelScope ks(s, "fused_pass", 2.0 * (double)s.size * sizeof(V2<R>));
                 if (lds8 > 160 * 1024) throw Error(QFVM_ERR_RUNTIME, "row-of-8 tile exceeds 160 KiB of LDS");
-                if (lds8 > 65536) allow_large_lds(reinterpret_cast<const void*>(&k_pass8<R>), s.device);
-                hipLaunchKernelGGL(k_pass8<R>, dim3((unsigned)tiles), dim3(threads8), lds8, s.stream, psi, pd, ops, druns);
+                const auto fn = pd.has_d4 ? k_pass8<R, true> : k_pass8<R, false>;
+                if (lds8 > 65536) allow_large_lds(reinterpret_cast<const void*>(fn), s.device);
+                hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(threads8), lds8, s.stream, psi, pd, ops, druns);
                 QFVM_LAUNCHED();
                 continue;
             } else {

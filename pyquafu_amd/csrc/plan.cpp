@@ -25,6 +25,7 @@ inline int popc(uint64_t x) { return __builtin_popcountll(x); }
 struct GInfo {
     uint64_t N = 0, D = 0;
     bool diag = false, multi = false, grad = false;
+    bool dense2 = false;  // a dense 2-target gate fused into row-of-8 groups (see plan_gates)
     cd d0, d1;
 };
 
@@ -66,6 +67,7 @@ struct GroupB {
 // dense 2x2 = 4; real and diagonal about 3; an X is mostly a free relabeling)
 int gate_cost(const Gate& g, const GInfo& gi) {
     if (gi.grad) return 2;
+    if (gi.dense2) return 16;  // a 4x4 on a row: four dense 2x2 gates' arithmetic
     if (gi.diag) return 3;
     switch (g.kind) {
         case G_SWAP: return 1;
@@ -113,6 +115,13 @@ FOpHost convert(const Gate& g, const GInfo& gi, const int S[4]) {
             o.cm_group |= 1u << j;
         else
             o.cm_ext |= 1ull << c;
+    }
+    if (gi.dense2) {
+        o.kind = F_DENSE4;
+        o.g = idx_in(g.targets[0]);
+        o.target = idx_in(g.targets[1]);
+        o.mx = g.mk;
+        return o;
     }
     if (gi.diag) {
         const int j = idx_in(g.target);
@@ -180,7 +189,14 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
     bool moved = false;  // phys differs from the identity
     static const std::vector<int> kIdentity;
     std::vector<GInfo> info(gates.size());
-    for (size_t i = 0; i < gates.size(); i++) info[i] = info_of(gates[i]);
+    for (size_t i = 0; i < gates.size(); i++) {
+        info[i] = info_of(gates[i]);
+        // row-of-8 groups hold a dense 2-target gate as one 4x4 op (both targets in one group)
+        if (gb == 3 && info[i].multi && gates[i].targets.size() == 2) {
+            info[i].multi = false;
+            info[i].dense2 = true;
+        }
+    }
     std::vector<char> done(gates.size(), 0);
     std::vector<PlanItem> plan;
     size_t first = 0;
@@ -296,8 +312,11 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
             for (int gi : G.gates) {
                 FOpHost o = convert(move_gate(gates[gi], cur), info[gi], fg.S);
                 o.src = gi;
+                // a 4x4 reads its two target bits as they are in memory: their pending relabelings
+                // are applied first, as for controls
+                const uint32_t flush_bits = o.cm_group | (o.kind == F_DENSE4 ? (1u << o.g) | (1u << o.target) : 0u);
                 for (int j = 0; j < FUSE_GROUP_BITS; j++) {
-                    if (((o.cm_group >> j) & 1) && pending(j)) {
+                    if (((flush_bits >> j) & 1) && pending(j)) {
                         FOpHost f{};
                         f.src = -1;
                         f.kind = F_FLUSH;
@@ -321,7 +340,7 @@ std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const Fu
                     // a pending relabeling of the same bit)
                     o.kind = F_DENSE;
                 }
-                if (o.kind != F_SWAP && o.kind != F_PHASE && pending(o.g)) {
+                if (o.kind != F_SWAP && o.kind != F_PHASE && o.kind != F_DENSE4 && pending(o.g)) {
                     o.flags |= FL_TARGET_PX | (((C >> o.g) & 1) ? FL_PX_CONST : 0);
                     o.pxmask = M[o.g];
                 }
@@ -495,6 +514,7 @@ void refresh_plan(std::vector<PlanItem>& plan, const std::vector<Gate>& gates) {
                         for (int k = 0; k < 4; k++) o.m[k] = c.m[k];
                         break;
                     }
+                    case F_DENSE4: o.mx = gates[(size_t)o.src].mk; break;
                     default: break;  // relabelings, flushes: no coefficients
                 }
             }

@@ -14,6 +14,7 @@ from tests.conftest import rand_state
 from tests.progs import random_gate_program
 
 F_DENSE, F_REAL, F_DIAG, F_SWAP, F_PHASE, F_FLUSH, F_GRAD, F_RELABEL = range(8)
+F_DENSE4 = 9
 G_MULTI = 4
 
 
@@ -36,6 +37,9 @@ def replay(plan, prog, n, psi):
     ops in order, on the positions the plan gives them."""
     psi = np.array(psi, dtype=complex)
     multi_rows = [r for r in prog.ops if r["code"] == 1]
+    if any(it["fused"] and it["group_bits"] == 3 for it in plan["items"]):
+        # row-of-8 plans run dense 2-target gates inside their groups (kind F_DENSE4)
+        multi_rows = [r for r in multi_rows if int(r["n_qubits"]) - int(r["n_ctrl"]) != 2]
     mi = 0
     idx = np.arange(1 << n, dtype=np.int64)
     for it in plan["items"]:
@@ -66,6 +70,13 @@ def replay(plan, prog, n, psi):
                 for j in range(gb):
                     if (op["cmg"] >> j) & 1:
                         cm |= 1 << S[j]
+                if op["kind"] == F_DENSE4:  # 4x4 on (S[g], S[g2]), index bit 0 <-> S[g]
+                    assert gb == 3 and len(m) == 16 and op["g"] != op["g2"]
+                    t = [S[op["g"]], S[op["g2"]]]
+                    assert not any((cm >> q) & 1 for q in t)
+                    ctrls = [q for q in range(n) if (cm >> q) & 1]
+                    psi = dense.apply_unitary(psi, n, ctrls, t, np.array(m).reshape(4, 4))
+                    continue
                 if op["kind"] == F_PHASE:
                     q = op["g"]
                     assert q not in S
