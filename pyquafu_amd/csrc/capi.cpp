@@ -1188,6 +1188,10 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
         // the group width qfvm_apply would use with default options (complex128: rows of 8)
         const int bb = std::min(cfg.tile_bits, n);
         if (tmp.dtype == QFVM_C128 && bb >= 10 && bb <= 13) cfg.group_bits = 3;
+        // complex64 gate lists with dense 2-target gates take the row-of-8 kernel (as qfvm_apply)
+        if (tmp.dtype == QFVM_C64 && bb >= 10 && bb <= 13)
+            for (const Gate& g : gates)
+                if (g.kind == G_MULTI && g.targets.size() == 2) cfg.group_bits = 3;
         cfg.layout = layout != 0 && cfg.group_bits == 3;
         std::vector<PlanItem> plan = plan_gates(gates, n, cfg);
         std::string j = "{\"items\": [";

@@ -572,7 +572,18 @@ void apply_gates(State& s, const std::vector<Gate>& gates) {
         for (const Gate& g : gates) launch_gate(s, g);
         return;
     }
-    const FuseConfig cfg = config_of(s, 1);
+    FuseConfig cfg = config_of(s, 1);
+    // complex64 runs rows of 16 by default; a gate list with dense 2-target gates takes the
+    // row-of-8 kernel instead, whose passes hold those gates (each would otherwise cost a sweep)
+    const int b = std::min(cfg.tile_bits, s.n);
+    if (cfg.group_bits != 3 && s.rows8 && s.dtype == QFVM_C64 && b >= 10 && b <= 13) {
+        for (const Gate& g : gates)
+            if (g.kind == G_MULTI && g.targets.size() == 2) {
+                cfg.group_bits = 3;
+                cfg.layout = s.layout;
+                break;
+            }
+    }
     if (s.plan_cache_on)
         execute_plan(s, cached_plan(s, gates, cfg));
     else

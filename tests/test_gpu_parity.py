@@ -418,7 +418,7 @@ def test_errors_are_reported():
 KERNELS = {"rows8": (("rows8", 1),), "split": (("rows8", 0), ("split_rows", 1)),
            "classic": (("rows8", 0), ("split_rows", 0)), "stream": (("rows8", 0), ("split_rows", 0), ("stream", 1)),
            "rows8c64": (("rows8", 1), ("rows8_c64", 1)), "layout": (("rows8", 1), ("layout", 1)),
-           "layoutc64": (("rows8", 1), ("rows8_c64", 1), ("layout", 1))}
+           "layoutc64": (("rows8", 1), ("rows8_c64", 1), ("layout", 1)), "default": ()}
 
 
 @pytest.mark.parametrize("dtype,tile,low,groups,kernel", [
@@ -530,11 +530,13 @@ def _two_qubit_dense_circuit(n, layers, seed):
     return qc
 
 
-@pytest.mark.parametrize("dtype,kernel", [("c128", "rows8"), ("c128", "layout"), ("c64", "rows8c64"), ("c128", "split")])
+@pytest.mark.parametrize("dtype,kernel", [("c128", "rows8"), ("c128", "layout"), ("c64", "rows8c64"), ("c64", "default"),
+                                          ("c128", "split")])
 def test_two_target_dense_gates_fused(dtype, kernel):
     """Dense 2-target gates (SWAP, iSWAP, RXX, RYY, controlled SWAP) run inside the row-of-8 passes
-    as 4x4 ops (no per-gate sweeps), against the oracle; the split-row kernel keeps them as per-gate
-    sweeps (multi_reg)."""
+    as 4x4 ops (no per-gate sweeps), against the oracle; complex64 with default options switches to
+    the row-of-8 kernel for such gate lists; the split-row kernel keeps them as per-gate sweeps
+    (multi_reg)."""
     n = 18
     prog = compile_circuit(_two_qubit_dense_circuit(n, 6, 77))
     cpu = CpuState(n)
