@@ -444,7 +444,40 @@ def dense_leg(n: int = 28, reps: int = 3):
                                         "GBps": round(2 * amp * 2 ** n / ms / 1e6, 1),
                                         "TFLOPs": round(8 * d * 2 ** n / ms / 1e9, 2)})
         dev.close()
+    out["two_target_circuit"] = two_target_leg(n)
     return out
+
+
+def two_target_leg(n: int, layers: int = 8, reps: int = 2):
+    """A circuit of dense 2-target gates (workloads.two_target_dense, PCG64 seed 20250007) on n qubits in both
+    precisions: those gates as 4x4 ops inside the row-of-8 passes (default options) against one
+    per-gate sweep each (option rows8 = 0). Wall time per circuit, best of `reps`."""
+    from pyquafu_amd import _native, workloads
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    prog = compile_circuit(workloads.two_target_dense(n, layers, 20250007))
+    res = {"workload": f"{n}q, {layers} layers of U3 + {n // 2} SWAP/iSWAP/RXX/RYY + 1 CSWAP", "gates": int(prog.n_ops)}
+    for dtype in ("c128", "c64"):
+        dev = _native.DeviceState(n, dtype)
+        row = {}
+        for name, rows8 in (("fused", 1), ("per_gate", 0)):
+            dev.set_option("rows8", rows8)
+            dev.init_zero()
+            dev.apply(prog)
+            dev.synchronize()
+            best = float("inf")
+            for _ in range(reps):
+                dev.profile(reset=True)
+                t0 = time.perf_counter()
+                dev.apply(prog)
+                dev.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            prof = dev.profile(reset=True)
+            row[name] = {"ms": round(best * 1e3, 2), "launches": {k: v["launches"] for k, v in prof.items()}}
+        row["speedup"] = round(row["per_gate"]["ms"] / row["fused"]["ms"], 2)
+        res[dtype] = row
+        dev.close()
+    return res
 
 
 def main(argv=None) -> int:

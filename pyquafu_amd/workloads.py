@@ -10,6 +10,9 @@ circuit on every machine:
   is the CPU-parity workload; 30 qubits, depth 40, 15 CRZ per layer the HBM workload.
 * `hea_ansatz(n, layers, seed)`: per layer RY on every qubit, RZ on every qubit, CZ(i, i+1 mod n);
   2*n*layers parameters ~ U[-pi, pi] drawn as one array, in gate order.
+* `two_target_dense(n, layers, seed)`: per layer U3 on every qubit (three angles per qubit as in
+  random_u3_cx, qubit by qubit), then n // 2 gates on random pairs cycling SWAP, iSWAP, RXX, RYY
+  (RXX / RYY draw their angle after the pair), then one controlled SWAP on three random qubits.
 * `tfim_hamiltonian(n, h)`: sum_i Z_i Z_{i+1 mod n} + h sum_i X_i.
 * `qft(n, x)`: X on the set bits of x, then the swap-free QFT in textbook order: for target j from
   n-1 down to 0, H(j) then CP(k -> j, 2 pi / 2^(j-k+1)) for k = j-1 .. 0. Output amplitude of |y> is
@@ -22,7 +25,8 @@ import numpy as np
 
 from .algorithms.hamiltonian import Hamiltonian, PauliOp
 from .circuits import QuantumCircuit
-from .elements import CPGate, CRZGate, CZGate, Parameter, RYGate, RZGate, U3Gate
+from .elements import (CPGate, CRZGate, CSwapGate, CZGate, ISwapGate, Parameter, RXXGate, RYGate, RYYGate, RZGate,
+                       SwapGate, U3Gate)
 
 
 def random_u3_cx(n: int, depth: int, crz_per_layer: int = 0, seed: int = 20250001, measure: bool = True):
@@ -41,6 +45,31 @@ def random_u3_cx(n: int, depth: int, crz_per_layer: int = 0, seed: int = 2025000
             qc << CRZGate(int(c), int(t), float(rng.uniform(0.0, 2 * np.pi)))
     if measure:
         qc.measure(list(range(n)))
+    return qc
+
+
+def two_target_dense(n: int, layers: int, seed: int = 20250007):
+    """Per layer: U3 on every qubit (angles uniform in [0, 2 pi)), then n // 2 gates on random qubit
+    pairs cycling SWAP, iSWAP, RXX(theta), RYY(theta), and one controlled SWAP: gates that lower to
+    dense 2-target matrices."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qc = QuantumCircuit(n)
+    for _ in range(layers):
+        for q in range(n):
+            qc << U3Gate(q, *map(float, rng.uniform(0, 2 * np.pi, 3)))
+        for k in range(n // 2):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            kind = k % 4
+            if kind == 0:
+                qc << SwapGate(a, b)
+            elif kind == 1:
+                qc << ISwapGate(a, b)
+            elif kind == 2:
+                qc << RXXGate(a, b, float(rng.uniform(0, 2 * np.pi)))
+            else:
+                qc << RYYGate(a, b, float(rng.uniform(0, 2 * np.pi)))
+        c, a, b = (int(x) for x in rng.choice(n, 3, replace=False))
+        qc << CSwapGate(c, a, b)
     return qc
 
 

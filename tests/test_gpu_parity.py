@@ -503,33 +503,6 @@ def test_layout_fewer_passes_and_standard_order_between_calls():
     assert launches[1] < launches[0], launches
 
 
-def _two_qubit_dense_circuit(n, layers, seed):
-    """U3 on every qubit, then SWAP / iSWAP / RXX / RYY on random pairs and a controlled SWAP per
-    layer: gates that lower to dense 2-target matrices."""
-    from pyquafu_amd.circuits import QuantumCircuit
-    from pyquafu_amd.elements import CSwapGate, ISwapGate, RXXGate, RYYGate, SwapGate, U3Gate
-
-    rng = np.random.Generator(np.random.PCG64(seed))
-    qc = QuantumCircuit(n)
-    for _ in range(layers):
-        for q in range(n):
-            qc << U3Gate(q, *map(float, rng.uniform(0, 2 * np.pi, 3)))
-        for k in range(n // 2):
-            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
-            kind = k % 4
-            if kind == 0:
-                qc << SwapGate(a, b)
-            elif kind == 1:
-                qc << ISwapGate(a, b)
-            elif kind == 2:
-                qc << RXXGate(a, b, float(rng.uniform(0, 2 * np.pi)))
-            else:
-                qc << RYYGate(a, b, float(rng.uniform(0, 2 * np.pi)))
-        c, a, b = (int(x) for x in rng.choice(n, 3, replace=False))
-        qc << CSwapGate(c, a, b)
-    return qc
-
-
 @pytest.mark.parametrize("dtype,kernel", [("c128", "rows8"), ("c128", "layout"), ("c64", "rows8c64"), ("c64", "default"),
                                           ("c128", "split")])
 def test_two_target_dense_gates_fused(dtype, kernel):
@@ -538,7 +511,7 @@ def test_two_target_dense_gates_fused(dtype, kernel):
     the row-of-8 kernel for such gate lists; the split-row kernel keeps them as per-gate sweeps
     (multi_reg)."""
     n = 18
-    prog = compile_circuit(_two_qubit_dense_circuit(n, 6, 77))
+    prog = compile_circuit(workloads.two_target_dense(n, 6, 77))
     cpu = CpuState(n)
     cpu.apply(prog)
     dev = gpu_state(n, dtype)
