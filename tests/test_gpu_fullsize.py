@@ -158,3 +158,34 @@ def test_qft_33q_closed_form_and_sampler(dtype):
         assert np.all(dist[off] < 1e-3)
     else:
         assert np.all(np.abs(idx - m) <= m * (2.0**-20 + 1e-6) + 1)
+
+
+def test_qft_34q_c64_closed_form_and_sampler():
+    """The largest state one MI355X holds in complex64: 34 qubits (128 GiB). QFT of a seeded basis
+    state: 10^5 sampled amplitudes against the closed form, 10^6 draws against floor(u 2^n / norm^2)
+    within the complex64 rounding bound (k additions of at most half an ulp of a sum below 1 each,
+    2^(n-53) cells per addition, plus the amplitudes' relative error)."""
+    n = 34
+    rng = np.random.Generator(np.random.PCG64(20250008))
+    x = int(rng.integers(0, 1 << n))
+    ys = rng.integers(0, 1 << n, size=100000, dtype=np.int64)
+    prog = compile_circuit(workloads.qft(n, x))
+    want = workloads.qft_amplitudes(n, x, ys)
+    print(f"[qft34 c64] {prog.n_ops} gates", flush=True)
+    dev = _native.DeviceState(n, "c64")
+    try:
+        dev.apply(prog)
+        got = dev.gather(ys)
+        err = float(np.abs(got - want).max())
+        print(f"[qft34 c64] max-abs over 10^5 sampled amplitudes vs closed form {err:.3e}", flush=True)
+        assert err < 1e-10
+        u = np.random.Generator(np.random.PCG64(20250009)).random(1000000)
+        idx = dev.sample(u).astype(np.int64)
+        total = dev.norm2()
+    finally:
+        dev.close()
+    cell = u * (float(1 << n) / total)
+    m = np.floor(cell).astype(np.int64)
+    print(f"[qft34 c64] norm^2 - 1 = {total - 1.0:.3e}; largest move {int(np.abs(idx - m).max())} cells", flush=True)
+    assert abs(total - 1.0) < 1e-5
+    assert np.all(np.abs(idx - m) <= m * (2.0 ** (n - 53) + 1e-6) + 1)
