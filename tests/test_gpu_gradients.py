@@ -372,3 +372,38 @@ def test_qnn_module_wrapper_forward_backward():
     out.sum().backward()
     assert model.weights.grad is not None and tuple(model.weights.grad.shape) == (1, n)
     assert np.isfinite(model.weights.grad.numpy()).all()
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-10), ("c64", 1e-4)])
+def test_adjoint_with_two_target_dense_gates(dtype, tol):
+    """RY / RZ parameters between SWAP, iSWAP and fixed-angle RXX gates (dense 2-target matrices):
+    the forward passes hold them as 4x4 ops (complex64 switches to the row-of-8 kernel for them),
+    the reverse sweep applies their inverses per gate; against the oracle's adjoint."""
+    from pyquafu_amd.algorithms.gradients.gradients import adjoint_program, hamiltonian_terms
+    from pyquafu_amd.circuits import QuantumCircuit
+    from pyquafu_amd.elements import ISwapGate, Parameter, RXXGate, RYGate, RZGate, SwapGate
+
+    n = 14
+    rng = np.random.Generator(np.random.PCG64(1500))
+    qc = QuantumCircuit(n)
+    k = 0
+    for layer in range(3):
+        for q in range(n):
+            qc << RYGate(q, Parameter(f"a{k}", float(rng.uniform(-np.pi, np.pi))))
+            qc << RZGate(q, Parameter(f"b{k}", float(rng.uniform(-np.pi, np.pi))))
+            k += 1
+        for q in range(layer % 2, n - 1, 2):
+            kind = (q // 2) % 3
+            if kind == 0:
+                qc << SwapGate(q, q + 1)
+            elif kind == 1:
+                qc << ISwapGate(q, q + 1)
+            else:
+                qc << RXXGate(q, q + 1, float(rng.uniform(0, 2 * np.pi)))
+    ham = workloads.tfim_hamiltonian(n)
+    prog, ops, gens, _ = adjoint_program(qc)
+    terms, coeffs = hamiltonian_terms(ham)
+    want, e_want = og.adjoint(prog, n, ops, gens, terms, coeffs)
+    got, e = engine_adjoint(prog, n, ops, gens, terms, coeffs, dtype=dtype)
+    assert abs(e - e_want) < tol
+    assert np.abs(got - want).max() < tol
