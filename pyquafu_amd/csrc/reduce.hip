@@ -12,6 +12,8 @@
 #include <mutex>
 #include <optional>
 
+#include <utility>
+
 #include "common.h"
 
 namespace qfvm {
@@ -162,8 +164,8 @@ __global__ void k_swap_states(V2<R>* __restrict__ psi, uint64_t size, uint64_t q
 // with the highest flip bit clear) are both in the tile, and its Z / Y mask only needs the global
 // index. Terms are packed first-fit into passes, so a Hamiltonian of Z strings and single-qubit X
 // terms on n qubits needs about (n - L) / (b - L) passes instead of one pass per term. Workgroups
-// walk a fixed set of tiles and add each tile's block sum to a per-term accumulator in tile order;
-// the per-workgroup accumulators are then added in index order (deterministic).
+// walk a fixed set of tiles; each thread adds its elements' contributions per term across them, the
+// workgroup reduces once, and the per-workgroup sums are added in index order (deterministic).
 // Algorithmic bytes per pass: one read of the state (s 2^n).
 constexpr int PT_MAX = 32;        // terms per launch (kernel argument)
 constexpr unsigned PT_GRID = 2048;
@@ -172,6 +174,7 @@ static_assert(RED_PARTIALS >= RED_GRID + 1, "reduction partials");
 
 struct PTerm {
     uint64_t z;     // Z / Y mask, global bits
+    uint32_t zl;    // the Z / Y bits inside the tile, as tile-local bits
     uint32_t f;     // flip mask in tile-local bits; 0 for a term of Z and I only
     int32_t fq;     // tile-local position of the highest flip bit
     int32_t y;      // number of Y letters
@@ -184,15 +187,25 @@ struct PPass {
     PTerm t[PT_MAX];
 };
 
-template <typename R>
+template <typename F, int... Ks>
+__device__ __forceinline__ void each_index(F&& f, std::integer_sequence<int, Ks...>) {
+    (f(std::integral_constant<int, Ks>{}), ...);
+}
+
+// One thread holds up to EPT elements of each tile (EPT = 2^b / RED_BLOCK at the largest tile) and
+// one accumulator per term across all of its workgroup's tiles; the workgroup reduces them once at
+// the end. A term's Z / Y parity splits into the tile base's (uniform, per tile) and the local
+// index's (32-bit, mask zl in local bits).
+template <typename R, int EPT>
 __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restrict__ psi, const PPass pp,
                                                            uint64_t tiles, double* __restrict__ partials) {
+    static_assert(EPT == 8 && RED_BLOCK == 256, "element i of a thread is local index tid + (i << 8)");
     extern __shared__ unsigned char lds_raw[];
     const int b = pp.b, L = pp.L, T = 1 << b;
     V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
-    uint64_t* hi = reinterpret_cast<uint64_t*>(tile + T);  // global offsets of the tile's high bits
+    double* wz = reinterpret_cast<double*>(tile + 2048);   // per thread: Walsh sums of its 8 probabilities
+    uint64_t* hi = reinterpret_cast<uint64_t*>(wz + 8 * RED_BLOCK);  // global offsets of the tile's high bits
     double* red = reinterpret_cast<double*>(hi + (1 << (b - L)));
-    double* acc = red + 8 * (RED_BLOCK / 64);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t lowmask = (1ull << L) - 1;
     for (int i = tid; i < (1 << (b - L)); i += RED_BLOCK) {
@@ -201,58 +214,103 @@ __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restri
             if ((i >> (k - L)) & 1) g |= 1ull << pp.q[k];
         hi[i] = g;
     }
-    if (tid < PT_MAX) acc[tid] = 0.0;
+    double a[PT_MAX];
+#pragma unroll
+    for (int k = 0; k < PT_MAX; k++) a[k] = 0.0;
     for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         uint64_t base = t;
         for (int k = 0; k < b; k++) {
             const uint64_t low = (1ull << pp.q[k]) - 1;
             base = (base & low) | ((base & ~low) << 1);
         }
+        // parity of each term's Z / Y bits outside the tile (bit k for term k; uniform)
+        uint32_t bpar = 0;
+        for (int k = 0; k < pp.nterms; k++) bpar |= (uint32_t)(__popcll(base & pp.t[k].z) & 1) << k;
         __syncthreads();  // the previous tile is no longer read; hi is built
-        for (int e = tid; e < T; e += RED_BLOCK) tile[e] = psi[base | ((uint64_t)e & lowmask) | hi[e >> L]];
+        V2<R> v[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = tid + i * (int)RED_BLOCK;
+            if (e < T) v[i] = psi[base | ((uint64_t)e & lowmask) | hi[e >> L]];
+        }
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = tid + i * (int)RED_BLOCK;
+            if (e < T) tile[e] = v[i];
+        }
         __syncthreads();
-        for (int c0 = 0; c0 < pp.nterms; c0 += 8) {
-            double a[8];
+        double pr[EPT];
 #pragma unroll
-            for (int k = 0; k < 8; k++) a[k] = 0.0;
-            for (int e = tid; e < T; e += RED_BLOCK) {
-                const uint64_t G = base | ((uint64_t)e & lowmask) | hi[e >> L];
-                const V2<R> x = tile[e];
-                const double p = (double)x.x * x.x + (double)x.y * x.y;
+        for (int i = 0; i < EPT; i++)
+            pr[i] = tid + i * (int)RED_BLOCK < T ? (double)v[i].x * v[i].x + (double)v[i].y * v[i].y : 0.0;
+        // Z strings: this thread's elements differ only in local bits 8..10 (= i), so a term's sum over
+        // them is (-1)^(parity of tid's bits and the tile base's) times the Walsh sum
+        // W[m] = sum_i (-1)^popc(i & m) pr[i] at m = the term's bits 8..10
+        {
+            double w[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    if (c0 + k < pp.nterms) {
-                        const PTerm& tm = pp.t[c0 + k];
-                        const int zpar = __popcll(G & tm.z) & 1;
-                        if (tm.f == 0) {
-                            a[k] += zpar ? -p : p;
-                        } else if (!((e >> tm.fq) & 1)) {
-                            const V2<R> w = tile[e ^ (int)tm.f];
-                            const double re = (double)x.x * w.x + (double)x.y * w.y;  // Re(x conj(w))
-                            const double im = (double)x.y * w.x - (double)x.x * w.y;  // Im(x conj(w))
-                            const int kk = (2 * zpar + tm.y) & 3;  // phase i^kk (PHASE_YZ = 1, i, -1, -i)
-                            const double v = kk == 0 ? re : kk == 1 ? -im : kk == 2 ? -re : im;
-                            a[k] += 2.0 * v;
-                        }
+            for (int m = 0; m < 8; m++) w[m] = pr[m];
+#pragma unroll
+            for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+                for (int m = 0; m < 8; m++)
+                    if (!(m & h)) {
+                        const double x = w[m], y = w[m | h];
+                        w[m] = x + y;
+                        w[m | h] = x - y;
                     }
+#pragma unroll
+            for (int m = 0; m < 8; m++) wz[m * RED_BLOCK + tid] = w[m];
+        }
+        // the term records are read inside the tile loop (an opaque zero offset keeps the compiler
+        // from holding all of them in scalar registers across it)
+        int k0 = 0;
+        asm volatile("" : "+s"(k0));
+        // one term at a time, the term index a compile-time constant (the accumulators stay in
+        // registers)
+        auto term = [&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if (k >= pp.nterms) return;
+            const PTerm& tm = pp.t[k + k0];
+            const uint32_t zl = tm.zl, f = tm.f;
+            const int fq = tm.fq, ty = tm.y;
+            const int bk = (int)(bpar >> k) & 1;
+            if (f == 0) {
+                const double wv = wz[(int)((zl >> 8) & 7u) * (int)RED_BLOCK + tid];
+                a[k] += ((__popc((uint32_t)tid & zl & 0xffu) ^ bk) & 1) ? -wv : wv;
+                return;
+            }
+#pragma unroll
+            for (int i = 0; i < EPT; i++) {
+                const int e = tid + i * (int)RED_BLOCK;
+                if (e >= T) continue;
+                const int zpar = (__popc((uint32_t)e & zl) ^ bk) & 1;
+                if (!((e >> fq) & 1)) {
+                    const V2<R> x = v[i], w = tile[e ^ (int)f];
+                    const double re = (double)x.x * w.x + (double)x.y * w.y;  // Re(x conj(w))
+                    const double im = (double)x.y * w.x - (double)x.x * w.y;  // Im(x conj(w))
+                    const int kk = (2 * zpar + ty) & 3;  // phase i^kk (PHASE_YZ = 1, i, -1, -i)
+                    const double val = kk == 0 ? re : kk == 1 ? -im : kk == 2 ? -re : im;
+                    a[k] += 2.0 * val;
                 }
             }
+        };
+        each_index(term, std::make_integer_sequence<int, PT_MAX>{});
+    }
+    // one reduction per workgroup: wavefront sums, then the waves in index order
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const double v = wave_sum(a[k]);
-                if (lane == 0) red[k * (RED_BLOCK / 64) + wave] = v;
-            }
-            __syncthreads();
-            if (tid < 8 && c0 + tid < pp.nterms) {
-                double v = 0.0;
-                for (int w = 0; w < (int)(RED_BLOCK / 64); w++) v += red[tid * (RED_BLOCK / 64) + w];
-                acc[c0 + tid] += v;
-            }
-            __syncthreads();
+    for (int k = 0; k < PT_MAX; k++) {
+        if (k < pp.nterms) {
+            const double w = wave_sum(a[k]);
+            if (lane == 0) red[k * (RED_BLOCK / 64) + wave] = w;
         }
     }
     __syncthreads();
-    if (tid < pp.nterms) partials[(uint64_t)tid * gridDim.x + blockIdx.x] = acc[tid];
+    if (tid < pp.nterms) {
+        double v = 0.0;
+        for (int w = 0; w < (int)(RED_BLOCK / 64); w++) v += red[tid * (RED_BLOCK / 64) + w];
+        partials[(uint64_t)tid * gridDim.x + blockIdx.x] = v;
+    }
 }
 
 // per-term sums of a launch's partials into out[term.slot]
@@ -310,7 +368,7 @@ void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* o
                         double* fallback, bool account) {
     if (n_terms > EXPECT_BATCH) throw Error(QFVM_ERR_VALUE, "too many terms in one expectation batch");
     const int n = s.n;
-    const int b = std::min(n, s.dtype == QFVM_C128 ? 11 : 12);
+    const int b = std::min(n, 11);  // 2^11 amplitudes per tile: 8 per thread (k_expect_tile<R, 8>)
     const int L = b < n ? std::min(5, b - 1) : b;
     const uint64_t lowmask = (1ull << L) - 1;
     const uint64_t tiles = 1ull << (n - b);
@@ -364,6 +422,9 @@ void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* o
                 const PauliHost& t = terms[order[c0 + k]];
                 PTerm& d = pp.t[k];
                 d.z = t.z;
+                d.zl = 0;
+                for (int q = 0; q < n; q++)
+                    if (((t.z >> q) & 1) && local_of[q] >= 0) d.zl |= 1u << local_of[q];
                 d.f = 0;
                 d.fq = 0;
                 for (int q = 0; q < n; q++)
@@ -374,15 +435,17 @@ void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* o
                 d.y = t.y;
                 d.slot = t.slot;
             }
-            const size_t lds = ((size_t)1 << b) * s.amp_bytes() + ((size_t)1 << (b - L)) * sizeof(uint64_t) +
-                               (8 * (RED_BLOCK / 64) + PT_MAX) * sizeof(double);
+            // a tile region of 2^11 amplitudes (smaller tiles leave part of it unused), the threads'
+            // Walsh sums, the high-bit offsets and the reduction slots
+            const size_t lds = (size_t)2048 * s.amp_bytes() + (size_t)8 * RED_BLOCK * sizeof(double) +
+                               ((size_t)1 << (b - L)) * sizeof(uint64_t) + (size_t)PT_MAX * (RED_BLOCK / 64) * sizeof(double);
             std::optional<KernelScope> ks;
             if (account) ks.emplace(s, "expect_tile", (double)s.size * s.amp_bytes());
             if (s.dtype == QFVM_C128)
-                hipLaunchKernelGGL(k_expect_tile<double>, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+                hipLaunchKernelGGL((k_expect_tile<double, 8>), dim3(grid), dim3(RED_BLOCK), lds, s.stream,
                                    reinterpret_cast<const double2*>(s.psi), pp, tiles, partials);
             else
-                hipLaunchKernelGGL(k_expect_tile<float>, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+                hipLaunchKernelGGL((k_expect_tile<float, 8>), dim3(grid), dim3(RED_BLOCK), lds, s.stream,
                                    reinterpret_cast<const float2*>(s.psi), pp, tiles, partials);
             QFVM_LAUNCHED();
             hipLaunchKernelGGL(k_term_sums, dim3((unsigned)pp.nterms), dim3(RED_BLOCK), 0, s.stream, partials, grid, pp,
