@@ -177,3 +177,73 @@ def test_ties_depend_on_parity():
     assert f == (0, 1)
     g = compose(f, f)
     assert g == (0, 1)  # even m stays; odd m: m + 1/2 -> m + 1 (even), then + 1/2 -> stays
+
+
+M_CROSS = (1 << 53) - 1
+
+
+def tree(xs):
+    """Composition in the order of the kernels' shuffle trees: neighbours first, then pairs of
+    pairs (len(xs) a power of two)."""
+    while len(xs) > 1:
+        xs = [compose(xs[i], xs[i + 1]) for i in range(0, len(xs), 2)]
+    return xs[0]
+
+
+def scan(xs):
+    """Inclusive scan in the order of the kernels' wavefront scans (Hillis-Steele: step d composes
+    the value d lanes back in front)."""
+    xs = list(xs)
+    d = 1
+    while d < len(xs):
+        xs = [compose(xs[i - d], xs[i]) if i >= d else xs[i] for i in range(len(xs))]
+        d *= 2
+    return xs
+
+
+def serial(xs):
+    acc = (0, 0)
+    for x in xs:
+        acc = compose(acc, x)
+    return acc
+
+
+def agree(m0, f, g):
+    a, b = m0 + f[m0 & 1], m0 + g[m0 & 1]
+    return a == b or (a >= M_CROSS and b >= M_CROSS)
+
+
+@pytest.mark.parametrize("kind", ["small", "ties", "near_cross", "saturated"])
+def test_parallel_orders_match_the_serial_fold(kind):
+    """The GPU composes element pairs in runs of 8, trees of 4 runs (a block) and of 64 blocks (a
+    tile), and scans tiles and blocks with wavefront scans. Saturation at 2^60 makes composition
+    associative only up to values past the binade; for an incoming m in [2^52, 2^53) every order
+    must give the serial fold's m exactly, or both must reach 2^53 - 1 (the block is replayed)."""
+    rng = np.random.Generator(np.random.PCG64(sum(map(ord, kind))))
+    e = -1  # S in [0.5, 1), ulp 2^-53
+    for _ in range(20):
+        n = 2048
+        if kind == "small":
+            p = rng.random(n) * 2.0**-40
+        elif kind == "ties":
+            p = (2 * rng.integers(0, 6, n) + 1) * 2.0**-54
+        elif kind == "near_cross":
+            p = rng.random(n) * 2.0**-11 / n * 8  # sums of a few blocks approach 2^53 ulps
+        else:
+            p = rng.random(n) * 2.0**-45
+            p[rng.integers(0, n, 3)] = 1.5  # beyond the binade: saturated elements
+        el = [xelem(float(v), e) for v in p]
+        runs = [serial(el[i:i + 8]) for i in range(0, n, 8)]
+        blocks = [tree(runs[i:i + 4]) for i in range(0, len(runs), 4)]
+        tile = tree(blocks)
+        whole = serial(el)
+        prefix = scan(blocks)
+        ends, acc = [], (0, 0)
+        for k, x in enumerate(el):  # the serial fold at every block end
+            acc = compose(acc, x)
+            if k % W == W - 1:
+                ends.append(acc)
+        for m0 in (1 << 52, (1 << 52) + 1, (1 << 53) - 2 - int(rng.integers(0, 1 << 40))):
+            assert agree(m0, tile, whole)
+            for b in range(len(blocks)):
+                assert agree(m0, prefix[b], ends[b])
