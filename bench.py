@@ -445,6 +445,7 @@ def dense_leg(n: int = 28, reps: int = 3):
                                         "TFLOPs": round(8 * d * 2 ** n / ms / 1e9, 2)})
         dev.close()
     out["two_target_circuit"] = two_target_leg(n)
+    out["dense_blocks"] = blocks_leg(n)
     return out
 
 
@@ -476,6 +477,54 @@ def two_target_leg(n: int, layers: int = 8, reps: int = 2):
             row[name] = {"ms": round(best * 1e3, 2), "launches": {k: v["launches"] for k, v in prof.items()}}
         row["speedup"] = round(row["per_gate"]["ms"] / row["fused"]["ms"], 2)
         res[dtype] = row
+        dev.close()
+    return res
+
+
+def blocks_leg(n: int, blocks=(5, 6), layer_counts=(1, 2, 4, 8, 16), reps: int = 2):
+    """Option dense_blocks on deep runs of gates on few qubits (workloads.deep_blocks, PCG64 seed
+    20250008: per layer U3 on every qubit, then a CX chain inside each block of k consecutive
+    qubits) on n qubits: the gates in fused passes (default options) against each block's run
+    collapsed into one dense k-target gate on the matrix cores (dense_blocks = k, dense_min_gates = 1,
+    so every block collapses), for 1 to 16 layers and both precisions. Wall time per circuit, the
+    host's collapse included, best of `reps`, and the kernels' HIP-event time per circuit (mean of
+    the `reps` calls)."""
+    from pyquafu_amd import _native, workloads
+    from pyquafu_amd.simulators.program import compile_circuit
+
+    res = {"workload": f"{n}q deep_blocks: per layer U3 on every qubit + CX chain in each block of k qubits",
+           "rows": []}
+    for dtype in ("c128", "c64"):
+        dev = _native.DeviceState(n, dtype)
+        dev.set_profiling(True)
+        for k in blocks:
+            for layers in layer_counts:
+                prog = compile_circuit(workloads.deep_blocks(n, k, layers))
+                row = {"dtype": dtype, "k": k, "layers": layers, "gates": int(prog.n_ops)}
+                for name, opts in (("passes", (("dense_blocks", 0),)),
+                                   ("dense", (("dense_blocks", k), ("dense_min_gates", 1)))):
+                    for key, v in opts:
+                        dev.set_option(key, v)
+                    dev.init_zero()
+                    dev.apply(prog)
+                    dev.synchronize()
+                    best = float("inf")
+                    for _ in range(reps):
+                        dev.profile(reset=True)
+                        t0 = time.perf_counter()
+                        dev.apply(prog)
+                        dev.synchronize()
+                        best = min(best, time.perf_counter() - t0)
+                    dev.profile(reset=True)
+                    for _ in range(reps):
+                        dev.apply(prog)
+                    dev.synchronize()
+                    prof = dev.profile(reset=True)
+                    row[name] = {"ms": round(best * 1e3, 2),
+                                 "kernel_ms": round(sum(v["ms"] for v in prof.values()) / reps, 2),
+                                 "launches": {key: v["launches"] // reps for key, v in prof.items()}}
+                row["speedup"] = round(row["passes"]["ms"] / row["dense"]["ms"], 2)
+                res["rows"].append(row)
         dev.close()
     return res
 

@@ -332,6 +332,17 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *   "stagger_shift" the blockIdx bit that selects the delayed half (0..16)
  *   "mfma"        1 = (default) dense gates on 4 to 8 targets run on the matrix cores (FP64 MFMA at
  *                 complex128, FP32 MFMA at complex64), 0 = on the vector ALUs
+ *   "dense_blocks" k (0 = off, or 4..8; default 5): qfvm_apply on states of 14 or more qubits with
+ *                 "mfma" on collapses runs of gates confined to at most k qubits into one dense gate
+ *                 on those qubits (the product of the run, formed on the host), run on the matrix
+ *                 cores; a run is collapsed when it acts on 4 or more qubits and holds at least
+ *                 "dense_min_gates" non-diagonal gates, and stands at the place of its last gate;
+ *                 every other gate runs as before. Pays off for deep runs on few qubits, whose
+ *                 gates would otherwise load the fused passes' arithmetic (brickwork circuits
+ *                 spread over all qubits never form such runs)
+ *   "dense_min_gates" the least number of non-diagonal gates a collapsed run holds (1..2^20,
+ *                 default 24: on 28 qubits the collapse wins from about 18 such gates at k = 5 and
+ *                 22 at k = 6; a 7- or 8-target gate costs 3 to 6.5 times a 5-target one)
  *   "checkpoints" K (0..64, default 0): qfvm_adjoint keeps up to K copies of the forward state at
  *                 op boundaries spread over the differentiated range and restores psi from them in
  *                 the reverse sweep instead of uncomputing through them; K shrinks to the copies that
@@ -353,11 +364,13 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
  * the ops in execution order: {"kind", "g", "g2", "cmg", "cme", "run", "m": [re, im] x 4, or x 16
  * for a 4x4 on group bits g and g2 (kind 9)}; other items
  * carry the gate on the positions it runs at. Lets the planner's reordering be checked without a
- * GPU. Arguments of 0 pick the defaults, as for qfvm_set_option; layout = 1 plans as option
- * "layout" does.
+ * GPU. Arguments of 0 pick the defaults, as for qfvm_set_option, except dense_blocks: 0 plans
+ * without collapsing runs, 4..8 collapses them as option "dense_blocks" does (at any n), with
+ * dense_min_gates (0 = 24); layout = 1 plans as option "layout" does. Items outside passes carry
+ * their matrix ("m": the 2x2, or the 2^k x 2^k of a dense gate, row-major).
  */
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
-                   int32_t pass_cost, int32_t layout,
+                   int32_t pass_cost, int32_t layout, int32_t dense_blocks, int32_t dense_min_gates,
                    const qfvm_program* prog, char* buf, int64_t cap, int64_t* len);
 
 #ifdef __cplusplus

@@ -101,21 +101,23 @@ SIGNATURES = {
     "qfvm_batch_destroy": (ctypes.c_int, [_vp]),
     "qfvm_batch_set_option": (ctypes.c_int, [_vp, _i32, ctypes.c_char_p, _i64]),
     "qfvm_batch_run": (ctypes.c_int, [_vp, ctypes.POINTER(CBatchItem), _i64]),
-    "qfvm_plan_json": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _i32, _i32, ctypes.POINTER(CProgram), ctypes.c_char_p, _i64,
+    "qfvm_plan_json": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, ctypes.POINTER(CProgram), ctypes.c_char_p, _i64,
                                       ctypes.POINTER(_i64)]),
 }
 
 
 def plan_json(prog: Program, n: int, dtype: str = "c128", tile_bits: int = 0, low_bits: int = 0,
-              max_groups: int = 0, pass_cost: int = 0, layout: bool = False) -> dict:
+              max_groups: int = 0, pass_cost: int = 0, layout: bool = False, dense_blocks: int = 0,
+              dense_min_gates: int = 0) -> dict:
     """The fused execution plan the engine would run for `prog` (host only; no GPU needed);
-    `layout` plans as option "layout" does."""
+    `layout`, `dense_blocks` and `dense_min_gates` plan as the options of those names do (with
+    `dense_blocks` = 0, the default here, no run is collapsed)."""
     import json  # pylint: disable=import-outside-toplevel
 
     cp, keep = as_cprogram(prog)
     ln = ctypes.c_int64(0)
     args = (int(n), _DTYPES[dtype], int(tile_bits), int(low_bits), int(max_groups), int(pass_cost), int(bool(layout)),
-            ctypes.byref(cp))
+            int(dense_blocks), int(dense_min_gates), ctypes.byref(cp))
     check(lib().qfvm_plan_json(*args, None, 0, ctypes.byref(ln)))
     buf = ctypes.create_string_buffer(ln.value + 1)
     check(lib().qfvm_plan_json(*args, buf, ln.value + 1, ctypes.byref(ln)))

@@ -1154,6 +1154,12 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
             s->graphs = value != 0;
         } else if (k == "mfma") {
             s->mfma = value != 0;
+        } else if (k == "dense_blocks") {
+            if (value != 0 && (value < 4 || value > 8)) throw Error(QFVM_ERR_VALUE, "dense_blocks must be 0 or 4..8");
+            s->dense_blocks = (int)value;
+        } else if (k == "dense_min_gates") {
+            if (value < 1 || value > (1 << 20)) throw Error(QFVM_ERR_VALUE, "dense_min_gates must be in 1..2^20");
+            s->dense_min_gates = (int)value;
         } else if (k == "checkpoints") {
             if (value < 0 || value > 64) throw Error(QFVM_ERR_VALUE, "checkpoints must be in 0..64");
             s->checkpoints = (int)value;
@@ -1167,7 +1173,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
 }
 
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,
-                   int32_t pass_cost, int32_t layout, const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
+                   int32_t pass_cost, int32_t layout, int32_t dense_blocks, int32_t dense_min_gates,
+                   const qfvm_program* prog, char* buf, int64_t cap, int64_t* len) {
     return guarded([&] {
         if (n < 0 || n > 34) throw Error(QFVM_ERR_VALUE, "n_qubits must be in [0, 34]");
         check_program(prog);
@@ -1180,6 +1187,9 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
             if (!is_gate_code(prog->ops[k].code)) throw Error(QFVM_ERR_VALUE, "unitary ops only");
             lower_op(tmp, prog, prog->ops[k], gates);
         }
+        if (dense_blocks != 0 && (dense_blocks < 4 || dense_blocks > 8))
+            throw Error(QFVM_ERR_VALUE, "dense_blocks must be 0 or 4..8");
+        if (dense_blocks) gates = collapse_dense_blocks(gates, dense_blocks, dense_min_gates > 0 ? dense_min_gates : 24);
         FuseConfig cfg;
         cfg.tile_bits = tile_bits > 0 ? tile_bits : default_tile_bits(tmp.dtype, 1);
         cfg.low_bits = low_bits > 0 ? low_bits : default_low_bits(tmp.dtype, 1);
@@ -1238,6 +1248,11 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
                 for (size_t k = 0; k < g.ctrls.size(); k++) j += (k ? ", " : "") + std::to_string(g.ctrls[k]);
                 j += "], \"targets\": [";
                 for (size_t k = 0; k < g.targets.size(); k++) j += (k ? ", " : "") + std::to_string(g.targets[k]);
+                j += "], \"m\": [";
+                if (g.kind == G_MULTI)
+                    for (size_t k = 0; k < g.mk.size(); k++) j += (k ? ", " : "") + cplx(g.mk[k]);
+                else
+                    for (int k = 0; k < 4; k++) j += (k ? ", " : "") + cplx(g.m[k]);
                 j += "]}";
             }
         }

@@ -156,6 +156,11 @@ struct State {
     // restored into psi during the reverse sweep instead of continuing the uncompute past them
     int checkpoints = 0;
     int mfma = 1;  // dense gates on 4 to 8 targets run on the matrix cores (option "mfma")
+    // runs of gates on at most dense_blocks qubits collapsed into one dense gate each, on the
+    // matrix cores (options "dense_blocks", 0 = off, and "dense_min_gates"; states of 14 or more
+    // qubits with "mfma" on)
+    int dense_blocks = 5;
+    int dense_min_gates = 24;
     std::vector<void*> ckpt;
     // graph mode (option "graphs"): qfvm_adjoint captures its stream work into a hipGraph and
     // replays it through an executable graph cached per program structure (see capi.cpp)

@@ -117,6 +117,11 @@ struct PlanItem {
 };
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg);
+// Runs of gates confined to at most k qubits (4 <= k <= 8), collapsed into one dense k-target gate
+// each (a G_MULTI on the run's qubits in ascending order, its matrix the product of the run's gates)
+// when the run holds at least min_gates non-diagonal gates and acts on 4 or more qubits; the
+// other gates keep their places (each collapsed gate stands at the place of its run's last gate).
+std::vector<Gate> collapse_dense_blocks(const std::vector<Gate>& gates, int k, int min_gates);
 // The plan of `gates` from the state's plan cache: a plan is reused for any gate list of the same
 // structure (kinds, qubits, controls, and the gate classes the planner reads: dense / real /
 // diagonal / exchange, zero diagonal entries), with its coefficients re-read from `gates`. Parameter

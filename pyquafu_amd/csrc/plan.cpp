@@ -168,6 +168,148 @@ void mark_diagonal_runs(FGroup& fg) {
 
 }  // namespace
 
+namespace {
+
+uint64_t support_of(const Gate& g) {
+    uint64_t m = g.cmask;
+    if (g.kind == G_MULTI)
+        for (int t : g.targets) m |= 1ull << t;
+    else
+        m |= 1ull << g.target;
+    return m;
+}
+
+// M <- g M for the 2^m x 2^m row-major M acting on qubits S (ascending; local bit j <-> S[j]): g is
+// applied to every column of M as to an m-qubit state.
+void apply_to_block(const Gate& g, const std::vector<int>& S, std::vector<cd>& M) {
+    const size_t d = (size_t)1 << S.size();
+    auto local = [&](int q) { return (size_t)1 << (std::find(S.begin(), S.end(), q) - S.begin()); };
+    size_t cm = 0;
+    for (int q = 0; q < 64; q++)
+        if ((g.cmask >> q) & 1) cm |= local(q);
+    if (g.kind == G_MULTI) {
+        const size_t k = g.targets.size(), dk = (size_t)1 << k;
+        std::vector<size_t> tb(k);
+        size_t tmask = 0;
+        for (size_t j = 0; j < k; j++) tmask |= tb[j] = local(g.targets[j]);
+        std::vector<size_t> at(dk);
+        std::vector<cd> in(dk);
+        for (size_t base = 0; base < d; base++) {
+            if ((base & tmask) || (base & cm) != cm) continue;
+            for (size_t r = 0; r < dk; r++) {
+                size_t i = base;
+                for (size_t j = 0; j < k; j++)
+                    if ((r >> j) & 1) i |= tb[j];
+                at[r] = i;
+            }
+            for (size_t col = 0; col < d; col++) {
+                for (size_t r = 0; r < dk; r++) in[r] = M[at[r] * d + col];
+                for (size_t r = 0; r < dk; r++) {
+                    cd acc = 0.0;
+                    for (size_t c = 0; c < dk; c++) acc += g.mk[r * dk + c] * in[c];
+                    M[at[r] * d + col] = acc;
+                }
+            }
+        }
+        return;
+    }
+    const size_t t = local(g.target);
+    for (size_t i0 = 0; i0 < d; i0++) {
+        if ((i0 & t) || (i0 & cm) != cm) continue;
+        cd* r0 = &M[i0 * d];
+        cd* r1 = &M[(i0 | t) * d];
+        for (size_t col = 0; col < d; col++) {
+            const cd a = r0[col], b = r1[col];
+            switch (g.kind) {
+                case G_DIAG:
+                    r0[col] = g.m[0] * a;
+                    r1[col] = g.m[1] * b;
+                    break;
+                case G_SWAP:
+                    r0[col] = b;
+                    r1[col] = a;
+                    break;
+                default:  // G_DENSE, G_REAL
+                    r0[col] = g.m[0] * a + g.m[1] * b;
+                    r1[col] = g.m[2] * a + g.m[3] * b;
+                    break;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+std::vector<Gate> collapse_dense_blocks(const std::vector<Gate>& gates, int k, int min_gates) {
+    // Open blocks have pairwise disjoint qubit sets; a gate joins (and merges) the blocks it shares
+    // qubits with when their union stays within k qubits, else those blocks are closed and the gate
+    // opens a block of its own. A gate h outside a block B but between B's gates shares no qubit
+    // with B's earlier gates (it would have joined B or closed it), so a collapsed block's product
+    // can stand at the place of its last gate; every other gate keeps its place.
+    struct Block {
+        uint64_t sup;
+        std::vector<int> idx;
+    };
+    std::vector<Block> open;
+    std::vector<int> owner(gates.size(), -1);  // collapsed block of each gate
+    std::vector<Gate> dense;                   // the collapsed blocks' gates
+    std::vector<int> last;                     // index of each collapsed block's last gate
+    auto close = [&](Block& b) {
+        int nd = 0;
+        for (int i : b.idx) nd += gates[i].kind != G_DIAG;
+        if (__builtin_popcountll(b.sup) < 4 || nd < min_gates) return;
+        std::sort(b.idx.begin(), b.idx.end());
+        Gate d;
+        d.kind = G_MULTI;
+        for (int q = 0; q < 64; q++)
+            if ((b.sup >> q) & 1) d.targets.push_back(q);
+        const size_t dim = (size_t)1 << d.targets.size();
+        d.mk.assign(dim * dim, cd(0.0));
+        for (size_t i = 0; i < dim; i++) d.mk[i * dim + i] = 1.0;
+        for (int i : b.idx) {
+            apply_to_block(gates[i], d.targets, d.mk);
+            owner[i] = (int)dense.size();
+        }
+        last.push_back(b.idx.back());
+        dense.push_back(std::move(d));
+    };
+    for (int i = 0; i < (int)gates.size(); i++) {
+        const Gate& g = gates[i];
+        if (g.kind == G_GRAD) throw Error(QFVM_ERR_RUNTIME, "dense blocks take unitary gates only");
+        const uint64_t gs = support_of(g);
+        uint64_t u = gs;
+        std::vector<size_t> touched;
+        for (size_t b = 0; b < open.size(); b++)
+            if (open[b].sup & gs) {
+                touched.push_back(b);
+                u |= open[b].sup;
+            }
+        Block nb{gs, {}};
+        if (__builtin_popcountll(u) <= k) {
+            nb.sup = u;
+            for (size_t t : touched) nb.idx.insert(nb.idx.end(), open[t].idx.begin(), open[t].idx.end());
+        } else {
+            for (size_t t : touched) close(open[t]);
+        }
+        for (size_t t = touched.size(); t-- > 0;) open.erase(open.begin() + (std::ptrdiff_t)touched[t]);
+        if (__builtin_popcountll(nb.sup) <= k) {
+            nb.idx.push_back(i);
+            open.push_back(std::move(nb));
+        }
+    }
+    for (Block& b : open) close(b);
+    if (dense.empty()) return gates;
+    std::vector<Gate> out;
+    out.reserve(gates.size());
+    for (int i = 0; i < (int)gates.size(); i++) {
+        if (owner[i] < 0)
+            out.push_back(gates[i]);
+        else if (last[owner[i]] == i)
+            out.push_back(dense[owner[i]]);
+    }
+    return out;
+}
+
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg) {
     const int b = std::min(cfg.tile_bits, n);
     const int gb = cfg.group_bits;
@@ -566,8 +708,14 @@ static FuseConfig config_of(const State& s, int states) {
     return cfg;
 }
 
-void apply_gates(State& s, const std::vector<Gate>& gates) {
-    if (gates.empty()) return;
+void apply_gates(State& s, const std::vector<Gate>& gates_in) {
+    if (gates_in.empty()) return;
+    // deep runs on few qubits: one dense gate per run on the matrix cores (below 2^14 amplitudes a
+    // launch costs more than the arithmetic it saves)
+    const bool collapse = s.dense_blocks && s.mfma && s.n >= 14;
+    std::vector<Gate> blocks;
+    if (collapse) blocks = collapse_dense_blocks(gates_in, s.dense_blocks, s.dense_min_gates);
+    const std::vector<Gate>& gates = collapse ? blocks : gates_in;
     if (!s.fusion || s.n < 6) {
         for (const Gate& g : gates) launch_gate(s, g);
         return;

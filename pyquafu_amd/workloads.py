@@ -13,6 +13,10 @@ circuit on every machine:
 * `two_target_dense(n, layers, seed)`: per layer U3 on every qubit (three angles per qubit as in
   random_u3_cx, qubit by qubit), then n // 2 gates on random pairs cycling SWAP, iSWAP, RXX, RYY
   (RXX / RYY draw their angle after the pair), then one controlled SWAP on three random qubits.
+* `deep_blocks(n, block, layers, seed)`: per layer, angles drawn as in random_u3_cx and U3 on every
+  qubit, then within each run of `block` consecutive qubits (0 .. block-1, block .. 2 block-1, ...;
+  the last run may be shorter) CX(q, q+1) down the run: deep runs of gates on few qubits, whose
+  gates never leave their run.
 * `tfim_hamiltonian(n, h)`: sum_i Z_i Z_{i+1 mod n} + h sum_i X_i.
 * `qft(n, x)`: X on the set bits of x, then the swap-free QFT in textbook order: for target j from
   n-1 down to 0, H(j) then CP(k -> j, 2 pi / 2^(j-k+1)) for k = j-1 .. 0. Output amplitude of |y> is
@@ -70,6 +74,19 @@ def two_target_dense(n: int, layers: int, seed: int = 20250007):
                 qc << RYYGate(a, b, float(rng.uniform(0, 2 * np.pi)))
         c, a, b = (int(x) for x in rng.choice(n, 3, replace=False))
         qc << CSwapGate(c, a, b)
+    return qc
+
+
+def deep_blocks(n: int, block: int, layers: int, seed: int = 20250008):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qc = QuantumCircuit(n)
+    for _ in range(layers):
+        ang = rng.uniform(0.0, 2 * np.pi, size=(n, 3))
+        for q in range(n):
+            qc << U3Gate(q, float(ang[q, 0]), float(ang[q, 1]), float(ang[q, 2]))
+        for b in range(0, n, block):
+            for q in range(b, min(b + block, n) - 1):
+                qc.cx(q, q + 1)
     return qc
 
 

@@ -528,12 +528,56 @@ def test_two_target_dense_gates_fused(dtype, kernel):
         assert "multi_reg" not in prof and prof["fused_pass"]["launches"] > 0
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("k,fusion", [(5, 1), (6, 1), (8, 1), (5, 0), (4, 1)])
+def test_dense_blocks_option(dtype, k, fusion):
+    """Option dense_blocks: deep runs on blocks of k qubits collapse into one dense k-target gate
+    per block (matrix cores), the rest (a shorter tail block) stays gates; against the oracle, with
+    and without fused passes, and one matrix-core launch per collapsed block."""
+    n = 18
+    prog = compile_circuit(workloads.deep_blocks(n, k, 6, seed=60 + k))
+    cpu = CpuState(n)
+    cpu.apply(prog)
+    dev = gpu_state(n, dtype)
+    dev.set_option("fusion", fusion)
+    dev.set_option("dense_blocks", k)
+    dev.set_option("dense_min_gates", 8)
+    dev.apply(prog)
+    got = dev.store()
+    prof = dev.profile()
+    dev.close()
+    print(f"[dense_blocks k={k} {dtype} fusion={fusion}] err {np.abs(got - cpu.store()).max():.3g}", flush=True)
+    assert np.abs(got - cpu.store()).max() < TOL[dtype]
+    assert prof["multi_mfma"]["launches"] == n // k + (1 if n % k >= 4 else 0)
+
+
+def test_dense_blocks_defaults():
+    """Default options collapse deep 5-qubit runs (dense_blocks 5, 24 non-diagonal gates); with
+    dense_blocks = 0, or mfma = 0, the same circuit runs in fused passes only."""
+    n = 16
+    prog = compile_circuit(workloads.deep_blocks(n, 5, 4, seed=71))
+    cpu = CpuState(n)
+    cpu.apply(prog)
+    for opts, dense in (((), True), ((("dense_blocks", 0),), False), ((("mfma", 0),), False)):
+        dev = gpu_state(n)
+        for key, v in opts:
+            dev.set_option(key, v)
+        dev.apply(prog)
+        got = dev.store()
+        prof = dev.profile()
+        dev.close()
+        assert np.abs(got - cpu.store()).max() < TOL["c128"]
+        assert ("multi_mfma" in prof) == dense and "multi_reg" not in prof and "multi_lds" not in prof
+
+
 def test_option_validation():
     dev = gpu_state(8)
     with pytest.raises(_native.EngineError):
         dev.set_option("tile_bits", 14)  # 2^14 complex128 amplitudes exceed the LDS
     with pytest.raises(_native.EngineError):
         dev.set_option("no_such_option", 1)
+    with pytest.raises(_native.EngineError):
+        dev.set_option("dense_blocks", 3)
 
 
 def test_gather_and_inner():

@@ -31,12 +31,13 @@ def move_bits(psi, n, Q, out_bit):
     return new
 
 
-def replay(plan, prog, n, psi):
+def replay(plan, prog, n, psi, collapsed=False):
     """Executes a plan's fused ops as controlled one-qubit gates on qubit positions, in plan order,
-    and each pass's tile permutation (layout-aware plans). Non-fused items are the program's MULTI
-    ops in order, on the positions the plan gives them."""
+    and each pass's tile permutation (layout-aware plans). Non-fused items are dense gates with the
+    matrix the plan gives them: the program's MULTI ops in order (their matrices checked against
+    the program's), or, for plans with collapsed runs of gates, whatever the collapse formed."""
     psi = np.array(psi, dtype=complex)
-    multi_rows = [r for r in prog.ops if r["code"] == 1]
+    multi_rows = [] if collapsed else [r for r in prog.ops if r["code"] == 1]
     if any(it["fused"] and it["group_bits"] == 3 for it in plan["items"]):
         # row-of-8 plans run dense 2-target gates inside their groups (kind F_DENSE4)
         multi_rows = [r for r in multi_rows if int(r["n_qubits"]) - int(r["n_ctrl"]) != 2]
@@ -44,14 +45,18 @@ def replay(plan, prog, n, psi):
     idx = np.arange(1 << n, dtype=np.int64)
     for it in plan["items"]:
         if not it["fused"]:
-            row = multi_rows[mi]
-            mi += 1
-            q = prog.idx[row["q_off"]: row["q_off"] + row["n_qubits"]].tolist()
-            nc = int(row["n_ctrl"])
-            assert len(it["ctrls"]) == nc and len(it["targets"]) == len(q) - nc
-            d = 1 << (len(q) - nc)
-            a = prog.mat[row["m_off"]: row["m_off"] + 2 * d * d]
-            psi = dense.apply_unitary(psi, n, it["ctrls"], it["targets"], (a[0::2] + 1j * a[1::2]).reshape(d, d))
+            assert it["kind"] == G_MULTI
+            d = 1 << len(it["targets"])
+            u = np.array([complex(*z) for z in it["m"]]).reshape(d, d)
+            if not collapsed:
+                row = multi_rows[mi]
+                mi += 1
+                q = prog.idx[row["q_off"]: row["q_off"] + row["n_qubits"]].tolist()
+                nc = int(row["n_ctrl"])
+                assert len(it["ctrls"]) == nc and len(it["targets"]) == len(q) - nc
+                a = prog.mat[row["m_off"]: row["m_off"] + 2 * d * d]
+                assert np.array_equal(u, (a[0::2] + 1j * a[1::2]).reshape(d, d))
+            psi = dense.apply_unitary(psi, n, it["ctrls"], it["targets"], u)
             continue
         Q = it["Q"]
         assert Q == sorted(Q) and len(set(Q)) == len(Q)
@@ -113,7 +118,7 @@ def check(prog, n, **kw):
     cpu = CpuState(n)
     cpu.load(psi0)
     cpu.apply(prog)
-    got = replay(plan, prog, n, psi0)
+    got = replay(plan, prog, n, psi0, collapsed=bool(kw.get("dense_blocks")))
     assert np.abs(got - cpu.store()).max() < 1e-12
     return plan
 
@@ -202,3 +207,36 @@ def test_layout_plans_replay_random_shapes(seed):
     plan = check(prog, n, tile_bits=tile, low_bits=low, max_groups=groups, layout=True)
     fused = [it for it in plan["items"] if it["fused"]]
     assert all(len(it["Q"]) == tile for it in fused)
+
+
+def test_dense_blocks_collapse_deep_runs():
+    """Deep runs on 5-qubit blocks collapse into one dense 5-target gate per block, on the block's
+    qubits in ascending order; the replayed plan reproduces the oracle."""
+    n = 13
+    prog = compile_circuit(workloads.deep_blocks(n, 5, 6))
+    plan = check(prog, n, dense_blocks=5, dense_min_gates=8)
+    dense_items = [it for it in plan["items"] if not it["fused"]]
+    assert sorted(it["targets"] for it in dense_items) == [[0, 1, 2, 3, 4], [5, 6, 7, 8, 9]]
+    # the 3-qubit tail stays gates (a dense gate needs 4 or more qubits), run in fused passes
+    assert any(it["fused"] for it in plan["items"])
+    # a threshold above the runs' gate count leaves the plan as it was
+    assert _native.plan_json(prog, n, dense_blocks=5, dense_min_gates=10**6) == _native.plan_json(prog, n)
+
+
+@pytest.mark.parametrize("k", [4, 5, 6, 8])
+@pytest.mark.parametrize("seed", range(4))
+def test_dense_blocks_random_programs(k, seed):
+    """Random gate programs (controls, diagonal and dense multi-target gates) with every qualifying
+    run collapsed: the order the collapse emits keeps the result."""
+    rng = np.random.Generator(np.random.PCG64(900 + seed))
+    n = int(rng.integers(7, 12))
+    prog = random_gate_program(n, 80, rng)
+    plan = check(prog, n, dense_blocks=k, dense_min_gates=int(rng.integers(1, 4)))
+    assert all(len(it["targets"]) <= max(k, 6) for it in plan["items"] if not it["fused"])
+
+
+def test_dense_blocks_option_range():
+    prog = compile_circuit(workloads.deep_blocks(8, 4, 2))
+    for bad in (1, 3, 9):
+        with pytest.raises(_native.EngineError):
+            _native.plan_json(prog, 8, dense_blocks=bad)
