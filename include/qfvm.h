@@ -332,8 +332,9 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *   "stagger_shift" the blockIdx bit that selects the delayed half (0..16)
  *   "mfma"        1 = (default) dense gates on 4 to 8 targets run on the matrix cores (FP64 MFMA at
  *                 complex128, FP32 MFMA at complex64), 0 = on the vector ALUs
- *   "dense_blocks" k (0 = off, or 4..8; default 5): qfvm_apply on states of 14 or more qubits with
- *                 "mfma" on collapses runs of gates confined to at most k qubits into one dense gate
+ *   "dense_blocks" k (0 = off, or 4..8; default 5): qfvm_apply and qfvm_run (not qfvm_adjoint) on
+ *                 states of 14 or more qubits with "mfma" on collapse runs of gates confined to at
+ *                 most k qubits into one dense gate
  *                 on those qubits (the product of the run, formed on the host), run on the matrix
  *                 cores; a run is collapsed when it acts on 4 or more qubits and holds at least
  *                 "dense_min_gates" non-diagonal gates, and stands at the place of its last gate;

@@ -276,7 +276,7 @@ struct Runner {
 
     void flush() {
         if (!pending.empty()) {
-            apply_gates(s, pending);
+            apply_circuit(s, pending);
             pending.clear();
         }
     }
@@ -836,7 +836,7 @@ int qfvm_apply(qfvm_handle h, const qfvm_program* prog) {
                 throw Error(QFVM_ERR_VALUE, "qfvm_apply takes unitary ops only (use qfvm_run for measure/reset/cif)");
             lower_op(*s, prog, prog->ops[k], gates);
         }
-        apply_gates(*s, gates);
+        apply_circuit(*s, gates);
         QFVM_HIP(hipStreamSynchronize(s->stream));
     });
 }

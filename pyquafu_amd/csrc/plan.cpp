@@ -708,14 +708,17 @@ static FuseConfig config_of(const State& s, int states) {
     return cfg;
 }
 
-void apply_gates(State& s, const std::vector<Gate>& gates_in) {
-    if (gates_in.empty()) return;
+void apply_circuit(State& s, const std::vector<Gate>& gates) {
     // deep runs on few qubits: one dense gate per run on the matrix cores (below 2^14 amplitudes a
     // launch costs more than the arithmetic it saves)
-    const bool collapse = s.dense_blocks && s.mfma && s.n >= 14;
-    std::vector<Gate> blocks;
-    if (collapse) blocks = collapse_dense_blocks(gates_in, s.dense_blocks, s.dense_min_gates);
-    const std::vector<Gate>& gates = collapse ? blocks : gates_in;
+    if (s.dense_blocks && s.mfma && s.n >= 14 && !gates.empty())
+        apply_gates(s, collapse_dense_blocks(gates, s.dense_blocks, s.dense_min_gates));
+    else
+        apply_gates(s, gates);
+}
+
+void apply_gates(State& s, const std::vector<Gate>& gates) {
+    if (gates.empty()) return;
     if (!s.fusion || s.n < 6) {
         for (const Gate& g : gates) launch_gate(s, g);
         return;

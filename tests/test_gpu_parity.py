@@ -570,6 +570,41 @@ def test_dense_blocks_defaults():
         assert ("multi_mfma" in prof) == dense and "multi_reg" not in prof and "multi_lds" not in prof
 
 
+def test_dense_blocks_in_run_between_measurements():
+    """qfvm_run collapses the gate runs between mid-circuit measurements (default options): the
+    deep-block circuit twice with measurements of qubits 2 and 7 in between, against the oracle's
+    run with the same uniforms."""
+    from pyquafu_amd.simulators.program import MEASURE, ProgramBuilder
+
+    n = 16
+    gates = compile_circuit(workloads.deep_blocks(n, 5, 4, seed=72))
+    b = ProgramBuilder()
+    for rep in range(2):
+        for r in gates.ops:
+            q = gates.idx[r["q_off"]: r["q_off"] + r["n_qubits"]].tolist()
+            m = None
+            if r["m_off"] >= 0:
+                d = 1 << (int(r["n_qubits"]) - int(r["n_ctrl"]))
+                a = gates.mat[r["m_off"]: r["m_off"] + 2 * d * d]
+                m = (a[0::2] + 1j * a[1::2]).reshape(d, d)
+            b.add(int(r["code"]), q, n_ctrl=int(r["n_ctrl"]), matrix=m, param=float(r["param"]))
+        if rep == 0:
+            b.add(MEASURE, [2], cbits=[0])
+            b.add(MEASURE, [7], cbits=[1])
+    prog = b.program(n_qubits=n, n_cbits=2)
+    u = np.array([0.3, 0.8])
+    cpu = CpuState(n)
+    want_creg = cpu.run(prog, u, 2)
+    dev = gpu_state(n)
+    got_creg = dev.run(prog, u, 2)
+    prof = dev.profile()
+    got = dev.store()
+    dev.close()
+    assert np.array_equal(got_creg, want_creg)
+    assert np.abs(got - cpu.store()).max() < TOL["c128"]
+    assert prof["multi_mfma"]["launches"] == 6  # three 5-qubit blocks per run of gates
+
+
 def test_option_validation():
     dev = gpu_state(8)
     with pytest.raises(_native.EngineError):
