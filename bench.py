@@ -12,6 +12,7 @@ Printed (one JSON line):
                  (torch neg_) over a buffer of the state's size, and frac_of_stream = achieved / its rate
   layout         the same circuit with the layout-aware planner (option "layout"): time per step,
                  fused passes, speedup over the default planner
+  merge_1q       runs of U3 layers with one-qubit runs merged (default) and not (option merge_1q)
   gate_equiv     the unfused per-gate byte model (sum over gates of 2 * 16 B * 2^(n - controls)) /
                  step time: comparable with the CPU, may exceed the HBM peak once gates are fused
   cpu_baseline   the CPU oracle (a restatement of qfvm's OpenMP engine, built with its flags) on
@@ -661,6 +662,25 @@ def main(argv=None) -> int:
         "pass_model_GBps": round(sum(v["bytes"] for v in lay_prof.values()) / args.steps / dt_lay / 1e9, 2),
     }
 
+    # option merge_1q (runs of one-qubit gates merged, default on) on runs of U3 layers
+    runs_leg = {"workload": f"{n}q {args.dtype} workloads.u3_runs: 8 layers of U3 on every qubit, CX(2i, 2i+1) "
+                            "after every other layer (PCG64 seed 20250009)"}
+    rprog = compile_circuit(workloads.u3_runs(n, 8, 2))
+    for merge in (1, 0):
+        dev.set_option("merge_1q", merge)
+        dev.apply(rprog)
+        dev.synchronize()
+        dev.profile(reset=True)
+        t0 = time.perf_counter()
+        dev.apply(rprog)
+        dev.synchronize()
+        rp = dev.profile(reset=True)
+        runs_leg["merged" if merge else "unmerged"] = {
+            "ms": round((time.perf_counter() - t0) * 1e3, 2),
+            "fused_passes": rp.get("fused_pass", {}).get("launches", 0)}
+    dev.set_option("merge_1q", 1)
+    runs_leg["speedup"] = round(runs_leg["unmerged"]["ms"] / runs_leg["merged"]["ms"], 2)
+
     ge = gate_equiv_bytes(prog, n, amp)
     out = {
         "metric": METRIC,
@@ -701,6 +721,7 @@ def main(argv=None) -> int:
         "norm_error": norm_err,
         "expect_tfim": expect_tfim,
         "layout": layout_leg,
+        "merge_1q": runs_leg,
     }
     dev.close()
     print("[bench] streaming reference", file=sys.stderr, flush=True)

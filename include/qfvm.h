@@ -341,6 +341,9 @@ int qfvm_profile_json(qfvm_handle h, char* buf, int64_t cap, int64_t* len, int32
  *                 every other gate runs as before. Pays off for deep runs on few qubits, whose
  *                 gates would otherwise load the fused passes' arithmetic (brickwork circuits
  *                 spread over all qubits never form such runs)
+ *   "merge_1q"    1 = (default) qfvm_apply and qfvm_run merge every run of uncontrolled one-qubit
+ *                 gates on one qubit (no other gate on that qubit in between) into one gate, the
+ *                 run's product, at the place of its last gate; 0 = off
  *   "dense_min_gates" the least number of non-diagonal gates a collapsed run holds (1..2^20,
  *                 default 24: on 28 qubits the collapse wins from about 18 such gates at k = 5 and
  *                 22 at k = 6; a 7- or 8-target gate costs 3 to 6.5 times a 5-target one)
@@ -367,7 +370,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value);
  * carry the gate on the positions it runs at. Lets the planner's reordering be checked without a
  * GPU. Arguments of 0 pick the defaults, as for qfvm_set_option, except dense_blocks: 0 plans
  * without collapsing runs, 4..8 collapses them as option "dense_blocks" does (at any n), with
- * dense_min_gates (0 = 24); layout = 1 plans as option "layout" does. Items outside passes carry
+ * dense_min_gates (0 = 24); layout = 1 plans as option "layout" does. One-qubit runs are merged
+ * as with option "merge_1q" on (its default). Items outside passes carry
  * their matrix ("m": the 2x2, or the 2^k x 2^k of a dense gate, row-major).
  */
 int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits, int32_t max_groups,

@@ -1157,6 +1157,8 @@ int qfvm_set_option(qfvm_handle h, const char* key, int64_t value) {
         } else if (k == "dense_blocks") {
             if (value != 0 && (value < 4 || value > 8)) throw Error(QFVM_ERR_VALUE, "dense_blocks must be 0 or 4..8");
             s->dense_blocks = (int)value;
+        } else if (k == "merge_1q") {
+            s->merge_1q = value != 0;
         } else if (k == "dense_min_gates") {
             if (value < 1 || value > (1 << 20)) throw Error(QFVM_ERR_VALUE, "dense_min_gates must be in 1..2^20");
             s->dense_min_gates = (int)value;
@@ -1187,8 +1189,11 @@ int qfvm_plan_json(int32_t n, int32_t dtype, int32_t tile_bits, int32_t low_bits
             if (!is_gate_code(prog->ops[k].code)) throw Error(QFVM_ERR_VALUE, "unitary ops only");
             lower_op(tmp, prog, prog->ops[k], gates);
         }
-        if (dense_blocks != 0 && (dense_blocks < 4 || dense_blocks > 8))
+        if (dense_blocks < 0 || (dense_blocks != 0 && (dense_blocks < 4 || dense_blocks > 8)))
             throw Error(QFVM_ERR_VALUE, "dense_blocks must be 0 or 4..8");
+        // as qfvm_apply: one-qubit runs merged (option merge_1q, on by default), then deep runs
+        // collapsed when asked
+        gates = merge_single_qubit_runs(gates);
         if (dense_blocks) gates = collapse_dense_blocks(gates, dense_blocks, dense_min_gates > 0 ? dense_min_gates : 24);
         FuseConfig cfg;
         cfg.tile_bits = tile_bits > 0 ? tile_bits : default_tile_bits(tmp.dtype, 1);

@@ -161,6 +161,7 @@ struct State {
     // qubits with "mfma" on)
     int dense_blocks = 5;
     int dense_min_gates = 24;
+    int merge_1q = 1;  // runs of one-qubit gates on one qubit merged first (option "merge_1q")
     std::vector<void*> ckpt;
     // graph mode (option "graphs"): qfvm_adjoint captures its stream work into a hipGraph and
     // replays it through an executable graph cached per program structure (see capi.cpp)

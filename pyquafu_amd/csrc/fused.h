@@ -117,6 +117,10 @@ struct PlanItem {
 };
 
 std::vector<PlanItem> plan_gates(const std::vector<Gate>& gates, int n, const FuseConfig& cfg);
+// Runs of uncontrolled one-qubit gates on one qubit (no other gate on that qubit between them)
+// merged into one gate each: diagonal if every factor is, real if every factor is real, else dense;
+// it stands at the place of the run's last gate and every other gate keeps its place.
+std::vector<Gate> merge_single_qubit_runs(const std::vector<Gate>& gates);
 // Runs of gates confined to at most k qubits (4 <= k <= 8), collapsed into one dense k-target gate
 // each (a G_MULTI on the run's qubits in ascending order, its matrix the product of the run's gates)
 // when the run holds at least min_gates non-diagonal gates and acts on 4 or more qubits; the

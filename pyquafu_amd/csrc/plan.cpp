@@ -240,6 +240,79 @@ void apply_to_block(const Gate& g, const std::vector<int>& S, std::vector<cd>& M
 
 }  // namespace
 
+std::vector<Gate> merge_single_qubit_runs(const std::vector<Gate>& gates) {
+    // A run is a sequence of uncontrolled one-target gates on one qubit with no other gate on that
+    // qubit between them; gates between the run's members act on other qubits, so the run's product
+    // can stand at the place of its last gate.
+    auto single = [](const Gate& g) {
+        return g.cmask == 0 && (g.kind == G_DENSE || g.kind == G_REAL || g.kind == G_DIAG || g.kind == G_SWAP);
+    };
+    std::vector<std::vector<int>> run(64);
+    std::vector<int> owner(gates.size(), -1);
+    std::vector<Gate> merged;
+    std::vector<int> last;
+    auto close = [&](int q) {
+        std::vector<int>& r = run[(size_t)q];
+        if (r.size() >= 2) {
+            cd M[4] = {1.0, 0.0, 0.0, 1.0};
+            bool all_diag = true, all_real = true;
+            for (int i : r) {
+                const Gate& g = gates[(size_t)i];
+                cd G[4];
+                if (g.kind == G_DIAG) {
+                    G[0] = g.m[0], G[1] = 0.0, G[2] = 0.0, G[3] = g.m[1];
+                } else if (g.kind == G_SWAP) {
+                    G[0] = 0.0, G[1] = 1.0, G[2] = 1.0, G[3] = 0.0;
+                } else {
+                    for (int j = 0; j < 4; j++) G[j] = g.m[j];
+                }
+                all_diag = all_diag && g.kind == G_DIAG;
+                all_real = all_real && (g.kind == G_REAL || g.kind == G_SWAP);
+                const cd P[4] = {G[0] * M[0] + G[1] * M[2], G[0] * M[1] + G[1] * M[3], G[2] * M[0] + G[3] * M[2],
+                                 G[2] * M[1] + G[3] * M[3]};
+                for (int j = 0; j < 4; j++) M[j] = P[j];
+            }
+            Gate m;
+            m.target = q;
+            if (all_diag) {
+                m.kind = G_DIAG;
+                m.m[0] = M[0];
+                m.m[1] = M[3];
+                m.m[2] = m.m[3] = 0.0;
+            } else {
+                m.kind = all_real ? G_REAL : G_DENSE;
+                for (int j = 0; j < 4; j++) m.m[j] = M[j];
+            }
+            for (int i : r) owner[(size_t)i] = (int)merged.size();
+            last.push_back(r.back());
+            merged.push_back(std::move(m));
+        }
+        r.clear();
+    };
+    for (int i = 0; i < (int)gates.size(); i++) {
+        const Gate& g = gates[(size_t)i];
+        if (single(g)) {
+            run[(size_t)g.target].push_back(i);
+            continue;
+        }
+        const uint64_t sup = support_of(g);
+        for (int q = 0; q < 64; q++)
+            if ((sup >> q) & 1) close(q);
+    }
+    for (int q = 0; q < 64; q++) close(q);
+    if (merged.empty()) return gates;
+    std::vector<Gate> out;
+    out.reserve(gates.size());
+    for (int i = 0; i < (int)gates.size(); i++) {
+        const int o = owner[(size_t)i];
+        if (o < 0)
+            out.push_back(gates[(size_t)i]);
+        else if (last[(size_t)o] == i)
+            out.push_back(merged[(size_t)o]);
+    }
+    return out;
+}
+
 std::vector<Gate> collapse_dense_blocks(const std::vector<Gate>& gates, int k, int min_gates) {
     // Open blocks have pairwise disjoint qubit sets; a gate joins (and merges) the blocks it shares
     // qubits with when their union stays within k qubits, else those blocks are closed and the gate
@@ -708,10 +781,15 @@ static FuseConfig config_of(const State& s, int states) {
     return cfg;
 }
 
-void apply_circuit(State& s, const std::vector<Gate>& gates) {
+void apply_circuit(State& s, const std::vector<Gate>& gates_in) {
+    if (gates_in.empty()) return;
+    // runs of one-qubit gates on one qubit: one gate each
+    std::vector<Gate> merged;
+    if (s.merge_1q) merged = merge_single_qubit_runs(gates_in);
+    const std::vector<Gate>& gates = s.merge_1q ? merged : gates_in;
     // deep runs on few qubits: one dense gate per run on the matrix cores (below 2^14 amplitudes a
     // launch costs more than the arithmetic it saves)
-    if (s.dense_blocks && s.mfma && s.n >= 14 && !gates.empty())
+    if (s.dense_blocks && s.mfma && s.n >= 14)
         apply_gates(s, collapse_dense_blocks(gates, s.dense_blocks, s.dense_min_gates));
     else
         apply_gates(s, gates);

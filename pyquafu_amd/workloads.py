@@ -17,6 +17,8 @@ circuit on every machine:
   qubit, then within each run of `block` consecutive qubits (0 .. block-1, block .. 2 block-1, ...;
   the last run may be shorter) CX(q, q+1) down the run: deep runs of gates on few qubits, whose
   gates never leave their run.
+* `u3_runs(n, layers, cx_every, seed)`: per layer, angles drawn as in random_u3_cx and U3 on every
+  qubit; after every `cx_every`-th layer CX(2i, 2i+1) for all i: runs of one-qubit gates.
 * `tfim_hamiltonian(n, h)`: sum_i Z_i Z_{i+1 mod n} + h sum_i X_i.
 * `qft(n, x)`: X on the set bits of x, then the swap-free QFT in textbook order: for target j from
   n-1 down to 0, H(j) then CP(k -> j, 2 pi / 2^(j-k+1)) for k = j-1 .. 0. Output amplitude of |y> is
@@ -87,6 +89,19 @@ def deep_blocks(n: int, block: int, layers: int, seed: int = 20250008):
         for b in range(0, n, block):
             for q in range(b, min(b + block, n) - 1):
                 qc.cx(q, q + 1)
+    return qc
+
+
+def u3_runs(n: int, layers: int, cx_every: int = 2, seed: int = 20250009):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qc = QuantumCircuit(n)
+    for layer in range(layers):
+        ang = rng.uniform(0.0, 2 * np.pi, size=(n, 3))
+        for q in range(n):
+            qc << U3Gate(q, float(ang[q, 0]), float(ang[q, 1]), float(ang[q, 2]))
+        if (layer + 1) % cx_every == 0:
+            for i in range(n // 2):
+                qc.cx(2 * i, 2 * i + 1)
     return qc
 
 

@@ -570,6 +570,26 @@ def test_dense_blocks_defaults():
         assert ("multi_mfma" in prof) == dense and "multi_reg" not in prof and "multi_lds" not in prof
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_single_qubit_runs_merged_on_gpu(dtype):
+    """Option merge_1q (default on) against off and the oracle: 8 layers of U3 on every qubit with
+    CX(2i, 2i+1) after every other layer; the merged circuit needs no more passes."""
+    n = 16
+    prog = compile_circuit(workloads.u3_runs(n, 8, 2, seed=81))
+    cpu = CpuState(n)
+    cpu.apply(prog)
+    passes = {}
+    for merge in (1, 0):
+        dev = gpu_state(n, dtype)
+        dev.set_option("merge_1q", merge)
+        dev.apply(prog)
+        got = dev.store()
+        passes[merge] = dev.profile()["fused_pass"]["launches"]
+        dev.close()
+        assert np.abs(got - cpu.store()).max() < TOL[dtype]
+    assert passes[1] <= passes[0]
+
+
 def test_dense_blocks_in_run_between_measurements():
     """qfvm_run collapses the gate runs between mid-circuit measurements (default options): the
     deep-block circuit twice with measurements of qubits 2 and 7 in between, against the oracle's

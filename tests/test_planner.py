@@ -240,3 +240,30 @@ def test_dense_blocks_option_range():
     for bad in (1, 3, 9):
         with pytest.raises(_native.EngineError):
             _native.plan_json(prog, 8, dense_blocks=bad)
+
+
+def test_single_qubit_runs_merged():
+    """Runs of one-qubit gates on one qubit (U3 layers, H / T / S / RX chains) become one op each in
+    the plan; a CX between them ends a run. The replayed plan reproduces the oracle."""
+    from pyquafu_amd.circuits import QuantumCircuit
+    from pyquafu_amd.elements import U3Gate
+
+    n = 9
+    rng = np.random.Generator(np.random.PCG64(5))
+    qc = QuantumCircuit(n)
+    for layer in range(6):
+        for q in range(n):
+            qc << U3Gate(q, *map(float, rng.uniform(0, 2 * np.pi, 3)))
+        if layer == 2:
+            qc.cx(0, 1)
+    for q in range(n):
+        qc.h(q)
+        qc.t(q)
+        qc.s(q)
+        qc.rz(q, 0.3)
+    prog = compile_circuit(qc)
+    plan = check(prog, n)
+    ops = [op for it in plan["items"] if it["fused"] for G in it["groups"] for op in G["ops"]
+           if op["kind"] not in (F_FLUSH, F_RELABEL)]
+    # qubits 0 and 1: runs before and after the CX (2 each); the others: one run each; plus the CX
+    assert len(ops) <= 2 * 2 + (n - 2) + 1
