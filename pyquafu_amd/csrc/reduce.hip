@@ -373,33 +373,51 @@ void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* o
     const uint64_t lowmask = (1ull << L) - 1;
     const uint64_t tiles = 1ull << (n - b);
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, PT_GRID);
-    bool done[EXPECT_BATCH];
-    int order[EXPECT_BATCH];
-    for (int i = 0; i < n_terms; i++) done[i] = false;
-    int left = n_terms;
-    while (left > 0) {
-        uint64_t Qh = 0;
-        int cnt = 0;
-        for (int i = 0; i < n_terms; i++) {
-            if (done[i]) continue;
-            const uint64_t hb = terms[i].flip & ~lowmask;
-            if (__builtin_popcountll(Qh | hb) <= b - L) {
-                Qh |= hb;
-                order[cnt++] = i;
-                done[i] = true;
-            }
-        }
-        if (cnt == 0) {  // the first remaining term cannot share a tile
-            for (int i = 0; i < n_terms; i++)
-                if (!done[i]) {
-                    queue_pauli_single(s, terms[i], fallback, out_dev + terms[i].slot, account);
-                    done[i] = true;
-                    left--;
-                    break;
-                }
+    // Passes: the terms with flips above the L low qubits are packed greedily into passes whose
+    // b - L chosen qubits hold all their high flips (a term too wide for any tile runs alone); the
+    // other terms (Z strings, flips on low qubits only) fit every pass and go to the pass holding the
+    // fewest terms, so that no pass is read for a handful of terms. Fixed-size arrays only: the
+    // allocation-free entry point runs this too.
+    int gid[EXPECT_BATCH];       // pass of each term (-1: runs alone)
+    uint64_t gQh[EXPECT_BATCH];  // high flip qubits of each pass
+    int gcnt[EXPECT_BATCH];
+    int ng = 0;
+    for (int i = 0; i < n_terms; i++) {
+        gid[i] = -1;
+        const uint64_t hb = terms[i].flip & ~lowmask;
+        if (hb == 0) continue;
+        if (__builtin_popcountll(hb) > b - L) {  // cannot share a tile
+            queue_pauli_single(s, terms[i], fallback, out_dev + terms[i].slot, account);
             continue;
         }
-        left -= cnt;
+        int g = 0;
+        while (g < ng && __builtin_popcountll(gQh[g] | hb) > b - L) g++;
+        if (g == ng) {
+            gQh[ng] = 0;
+            gcnt[ng++] = 0;
+        }
+        gQh[g] |= hb;
+        gcnt[g]++;
+        gid[i] = g;
+    }
+    for (int i = 0; i < n_terms; i++) {
+        if ((terms[i].flip & ~lowmask) != 0) continue;
+        if (ng == 0) {
+            gQh[0] = 0;
+            gcnt[ng++] = 0;
+        }
+        int g = 0;
+        for (int c = 1; c < ng; c++)
+            if (gcnt[c] < gcnt[g]) g = c;
+        gcnt[g]++;
+        gid[i] = g;
+    }
+    int order[EXPECT_BATCH];
+    for (int grp = 0; grp < ng; grp++) {
+        uint64_t Qh = gQh[grp];
+        int cnt = 0;
+        for (int i = 0; i < n_terms; i++)
+            if (gid[i] == grp) order[cnt++] = i;
         for (int q = L; q < n && __builtin_popcountll(Qh) < b - L; q++) Qh |= 1ull << q;
         PPass pp{};
         pp.b = b;
