@@ -108,8 +108,8 @@ def test_many_term_expectations(n, dtype):
           f"{ {k: v['launches'] for k, v in prof.items()} }", flush=True)
     assert err < (1e-12 if dtype == "c128" else 1e-5)
     if n == 20:
-        # the TFIM alone: the Z strings and the X terms on the tile's 5 low qubits and 6 (complex128)
-        # or 7 (complex64) chosen ones share the first pass; the remaining X terms take two more
+        # the TFIM alone: the X terms on qubits 5..19 fit 6 chosen tile qubits per pass (three
+        # passes); the Z strings and the X terms on the 5 low qubits fill those passes
         dev.profile(reset=True)
         got = dev.expect(tfim_terms(n))
         prof = dev.profile(reset=True)
