@@ -196,15 +196,19 @@ __device__ __forceinline__ void each_index(F&& f, std::integer_sequence<int, Ks.
 // one accumulator per term across all of its workgroup's tiles; the workgroup reduces them once at
 // the end. A term's Z / Y parity splits into the tile base's (uniform, per tile) and the local
 // index's (32-bit, mask zl in local bits).
-template <typename R, int EPT>
-__global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restrict__ psi, const PPass pp,
+template <typename R, int EPT, int NT>
+__global__ void __launch_bounds__(RED_BLOCK, NT <= 16 ? 4 : 1) k_expect_tile(const V2<R>* __restrict__ psi, const PPass pp,
                                                            uint64_t tiles, double* __restrict__ partials) {
     static_assert(EPT == 8 && RED_BLOCK == 256, "element i of a thread is local index tid + (i << 8)");
     extern __shared__ unsigned char lds_raw[];
     const int b = pp.b, L = pp.L, T = 1 << b;
+    // Two layouts: with up to 16 terms (WREG) the Walsh sums stay in registers and the thread's own
+    // elements are read back from the tile, so that 4 wavefronts fit per SIMD (118 VGPRs, 33.5 KiB
+    // of LDS); with up to 32 the Walsh sums go to LDS and the elements stay in registers.
+    constexpr bool WREG = NT <= 16;
     V2<R>* tile = reinterpret_cast<V2<R>*>(lds_raw);
-    double* wz = reinterpret_cast<double*>(tile + 2048);   // per thread: Walsh sums of its 8 probabilities
-    uint64_t* hi = reinterpret_cast<uint64_t*>(wz + 8 * RED_BLOCK);  // global offsets of the tile's high bits
+    double* wz = reinterpret_cast<double*>(tile + 2048);  // !WREG: per thread, Walsh sums of its 8 probabilities
+    uint64_t* hi = reinterpret_cast<uint64_t*>(wz + (WREG ? 0 : 8 * RED_BLOCK));  // global offsets of the high bits
     double* red = reinterpret_cast<double*>(hi + (1 << (b - L)));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t lowmask = (1ull << L) - 1;
@@ -214,9 +218,10 @@ __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restri
             if ((i >> (k - L)) & 1) g |= 1ull << pp.q[k];
         hi[i] = g;
     }
-    double a[PT_MAX];
+    static_assert(NT <= PT_MAX, "terms per launch");
+    double a[NT];
 #pragma unroll
-    for (int k = 0; k < PT_MAX; k++) a[k] = 0.0;
+    for (int k = 0; k < NT; k++) a[k] = 0.0;
     for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         uint64_t base = t;
         for (int k = 0; k < b; k++) {
@@ -246,8 +251,8 @@ __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restri
         // Z strings: this thread's elements differ only in local bits 8..10 (= i), so a term's sum over
         // them is (-1)^(parity of tid's bits and the tile base's) times the Walsh sum
         // W[m] = sum_i (-1)^popc(i & m) pr[i] at m = the term's bits 8..10
+        double w[8];
         {
-            double w[8];
 #pragma unroll
             for (int m = 0; m < 8; m++) w[m] = pr[m];
 #pragma unroll
@@ -259,8 +264,10 @@ __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restri
                         w[m] = x + y;
                         w[m | h] = x - y;
                     }
+            if constexpr (!WREG) {
 #pragma unroll
-            for (int m = 0; m < 8; m++) wz[m * RED_BLOCK + tid] = w[m];
+                for (int m = 0; m < 8; m++) wz[m * RED_BLOCK + tid] = w[m];
+            }
         }
         // the term records are read inside the tile loop (an opaque zero offset keeps the compiler
         // from holding all of them in scalar registers across it)
@@ -276,7 +283,17 @@ __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restri
             const int fq = tm.fq, ty = tm.y;
             const int bk = (int)(bpar >> k) & 1;
             if (f == 0) {
-                const double wv = wz[(int)((zl >> 8) & 7u) * (int)RED_BLOCK + tid];
+                const uint32_t zh = (zl >> 8) & 7u;
+                double wv;
+                if constexpr (WREG) {  // uniform: a select tree over the registers
+                    const bool s0 = zh & 1u, s1 = zh & 2u, s2 = zh & 4u;
+                    const double w01 = s0 ? w[1] : w[0], w23 = s0 ? w[3] : w[2];
+                    const double w45 = s0 ? w[5] : w[4], w67 = s0 ? w[7] : w[6];
+                    const double w03 = s1 ? w23 : w01, w47 = s1 ? w67 : w45;
+                    wv = s2 ? w47 : w03;
+                } else {
+                    wv = wz[(int)zh * (int)RED_BLOCK + tid];
+                }
                 a[k] += ((__popc((uint32_t)tid & zl & 0xffu) ^ bk) & 1) ? -wv : wv;
                 return;
             }
@@ -286,20 +303,20 @@ __global__ void __launch_bounds__(RED_BLOCK) k_expect_tile(const V2<R>* __restri
                 if (e >= T) continue;
                 const int zpar = (__popc((uint32_t)e & zl) ^ bk) & 1;
                 if (!((e >> fq) & 1)) {
-                    const V2<R> x = v[i], w = tile[e ^ (int)f];
-                    const double re = (double)x.x * w.x + (double)x.y * w.y;  // Re(x conj(w))
-                    const double im = (double)x.y * w.x - (double)x.x * w.y;  // Im(x conj(w))
+                    const V2<R> x = WREG ? tile[e] : v[i], y = tile[e ^ (int)f];
+                    const double re = (double)x.x * y.x + (double)x.y * y.y;  // Re(x conj(y))
+                    const double im = (double)x.y * y.x - (double)x.x * y.y;  // Im(x conj(y))
                     const int kk = (2 * zpar + ty) & 3;  // phase i^kk (PHASE_YZ = 1, i, -1, -i)
                     const double val = kk == 0 ? re : kk == 1 ? -im : kk == 2 ? -re : im;
                     a[k] += 2.0 * val;
                 }
             }
         };
-        each_index(term, std::make_integer_sequence<int, PT_MAX>{});
+        each_index(term, std::make_integer_sequence<int, NT>{});
     }
     // one reduction per workgroup: wavefront sums, then the waves in index order
 #pragma unroll
-    for (int k = 0; k < PT_MAX; k++) {
+    for (int k = 0; k < NT; k++) {
         if (k < pp.nterms) {
             const double w = wave_sum(a[k]);
             if (lane == 0) red[k * (RED_BLOCK / 64) + wave] = w;
@@ -453,18 +470,24 @@ void queue_expectations(State& s, const PauliHost* terms, int n_terms, double* o
                 d.y = t.y;
                 d.slot = t.slot;
             }
-            // a tile region of 2^11 amplitudes (smaller tiles leave part of it unused), the threads'
-            // Walsh sums, the high-bit offsets and the reduction slots
-            const size_t lds = (size_t)2048 * s.amp_bytes() + (size_t)8 * RED_BLOCK * sizeof(double) +
+            // up to 16 terms: the instance with 16 accumulators and the Walsh sums in registers,
+            // which fits four wavefronts per SIMD. LDS: a tile region of 2^11 amplitudes (smaller
+            // tiles leave part of it unused), the threads' Walsh sums (more than 16 terms), the
+            // high-bit offsets and the reduction slots
+            const bool few = pp.nterms <= 16;
+            const size_t lds = (size_t)2048 * s.amp_bytes() + (few ? 0 : (size_t)8 * RED_BLOCK * sizeof(double)) +
                                ((size_t)1 << (b - L)) * sizeof(uint64_t) + (size_t)PT_MAX * (RED_BLOCK / 64) * sizeof(double);
             std::optional<KernelScope> ks;
             if (account) ks.emplace(s, "expect_tile", (double)s.size * s.amp_bytes());
-            if (s.dtype == QFVM_C128)
-                hipLaunchKernelGGL((k_expect_tile<double, 8>), dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+            if (s.dtype == QFVM_C128) {
+                const auto fn = few ? k_expect_tile<double, 8, 16> : k_expect_tile<double, 8, PT_MAX>;
+                hipLaunchKernelGGL(fn, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
                                    reinterpret_cast<const double2*>(s.psi), pp, tiles, partials);
-            else
-                hipLaunchKernelGGL((k_expect_tile<float, 8>), dim3(grid), dim3(RED_BLOCK), lds, s.stream,
+            } else {
+                const auto fn = few ? k_expect_tile<float, 8, 16> : k_expect_tile<float, 8, PT_MAX>;
+                hipLaunchKernelGGL(fn, dim3(grid), dim3(RED_BLOCK), lds, s.stream,
                                    reinterpret_cast<const float2*>(s.psi), pp, tiles, partials);
+            }
             QFVM_LAUNCHED();
             hipLaunchKernelGGL(k_term_sums, dim3((unsigned)pp.nterms), dim3(RED_BLOCK), 0, s.stream, partials, grid, pp,
                                out_dev);
