@@ -551,6 +551,34 @@ def test_dense_blocks_option(dtype, k, fusion):
     assert prof["multi_mfma"]["launches"] == n // k + (1 if n % k >= 4 else 0)
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_dense_blocks_with_layout_planner(dtype):
+    """Collapsed blocks between layout-aware passes (option layout): the dense gates run on the
+    qubits' current positions; deep 5-qubit blocks then a brickwork spreading over all qubits, twice,
+    against the oracle."""
+    from pyquafu_amd.circuits import QuantumCircuit
+
+    n = 18
+    qc = QuantumCircuit(n)
+    for rep in range(2):
+        for g in workloads.deep_blocks(n, 5, 4, seed=90 + rep).gates:
+            qc << g
+        for g in workloads.random_u3_cx(n, 2, 3, seed=95 + rep, measure=False).gates:
+            qc << g
+    prog = compile_circuit(qc)
+    cpu = CpuState(n)
+    cpu.apply(prog)
+    dev = gpu_state(n, dtype)
+    for key, v in (("tile_bits", 10), ("layout", 1)) + ((("rows8_c64", 1),) if dtype == "c64" else ()):
+        dev.set_option(key, v)
+    dev.apply(prog)
+    got = dev.store()
+    prof = dev.profile()
+    dev.close()
+    assert np.abs(got - cpu.store()).max() < TOL[dtype]
+    assert prof["multi_mfma"]["launches"] == 6
+
+
 def test_dense_blocks_defaults():
     """Default options collapse deep 5-qubit runs (dense_blocks 5, 24 non-diagonal gates); with
     dense_blocks = 0, or mfma = 0, the same circuit runs in fused passes only."""

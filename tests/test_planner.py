@@ -267,3 +267,21 @@ def test_single_qubit_runs_merged():
            if op["kind"] not in (F_FLUSH, F_RELABEL)]
     # qubits 0 and 1: runs before and after the CX (2 each); the others: one run each; plus the CX
     assert len(ops) <= 2 * 2 + (n - 2) + 1
+
+
+def test_dense_blocks_in_layout_plans():
+    """Collapsed blocks between layout-aware passes run on the qubits' current positions: deep
+    5-qubit blocks, then a brickwork over all qubits, twice; the replayed plan reproduces the
+    oracle."""
+    from pyquafu_amd.circuits import QuantumCircuit
+
+    n = 14
+    qc = QuantumCircuit(n)
+    for rep in range(2):
+        for g in workloads.deep_blocks(n, 5, 4, seed=90 + rep).gates:
+            qc << g
+        for g in workloads.random_u3_cx(n, 2, 3, seed=95 + rep, measure=False).gates:
+            qc << g
+    plan = check(compile_circuit(qc), n, tile_bits=10, layout=True, dense_blocks=5)
+    # per repetition: qubits 0..4, 5..9 and the 4-qubit tail 10..13
+    assert sum(1 for it in plan["items"] if not it["fused"]) == 6
